@@ -1,0 +1,164 @@
+"""helioselene_amd -- MI355X (gfx950) variable-time multi-scalar multiplication for the
+Helios / Selene curve cycle, behind the C ABI of include/hs_gpu_msm.h.
+
+This module is a thin ctypes binding of helioselene_amd/_lib/libhs_gpu.so.  It mirrors the
+reference's point-level MSM call (HeliosPoint::multi_scalar_mul / helios_msm_vartime,
+include/helioselene_point.h:166, helios/include/helios_msm_vartime.h:43-72): inputs are the
+reference's byte formats (32-byte scalars, 120-byte radix-2^51 Jacobian points), outputs the
+32-byte compressed encoding or the Jacobian struct.  There is no CPU fallback: if the HIP
+library is missing or fails to load, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libhs_gpu.so")
+
+HELIOS, SELENE = 0, 1
+CURVES = {"helios": HELIOS, "selene": SELENE, HELIOS: HELIOS, SELENE: SELENE}
+EXACT = 1
+
+# group orders and coordinate moduli (reference helios_constants.h:57-64, selene_constants.h:56-64)
+P = (1 << 255) - 19
+Q = 0x7FFFFFFFFFFFFFFFFFFFFFFFFFFFFFFFBF7F782CB7656B586EB6D2727927C79F
+ORDER = {HELIOS: Q, SELENE: P}
+FIELD = {HELIOS: P, SELENE: Q}
+
+_lib = None
+
+
+class HsGpuError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded HIP library (raises when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HsGpuError(f"HIP library not built: {LIB_PATH} (run `python -m helioselene_amd.build`)")
+        L = ctypes.CDLL(LIB_PATH)
+        sz, vp, cp = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p
+        L.hs_gpu_msm_ex.argtypes = [ctypes.c_int, vp, vp, vp, vp, sz, ctypes.c_uint]
+        L.hs_gpu_msm_device.argtypes = [ctypes.c_int, vp, vp, sz, ctypes.c_uint, vp, vp, vp]
+        L.hs_gpu_helios_msm_vartime.argtypes = [vp, vp, vp, sz]
+        L.hs_gpu_selene_msm_vartime.argtypes = [vp, vp, vp, sz]
+        L.hs_gpu_last_error.restype = cp
+        L.hs_gpu_version.restype = cp
+        L.hs_gpu_window_bits.argtypes = [ctypes.c_int, sz]
+        L.hs_gpu_reference_window_bits.argtypes = [sz]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HsGpuError(f"hs_gpu error {rc}: {lib().hs_gpu_last_error().decode()}")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------- format helpers (host)
+def fe_to_r51(v):
+    m = (1 << 51) - 1
+    return [(v >> (51 * i)) & m for i in range(5)]
+
+
+def _words_to_r51(w):
+    """uint64 words (n, 4) of values < 2^256 -> radix-2^51 limbs (n, 5) (top limb keeps bits 204+)."""
+    m = np.uint64((1 << 51) - 1)
+    out = np.empty((w.shape[0], 5), dtype=np.uint64)
+    s = np.uint64
+    out[:, 0] = w[:, 0] & m
+    out[:, 1] = ((w[:, 0] >> s(51)) | (w[:, 1] << s(13))) & m
+    out[:, 2] = ((w[:, 1] >> s(38)) | (w[:, 2] << s(26))) & m
+    out[:, 3] = ((w[:, 2] >> s(25)) | (w[:, 3] << s(39))) & m
+    out[:, 4] = w[:, 3] >> s(12)
+    return out
+
+
+def jacobian_array(points_xy):
+    """[(x, y) or None] affine points (ints) -> uint64 array (n, 15) in the reference struct
+    layout (Z = 1, or X = Y = 1, Z = 0 for the identity)."""
+    n = len(points_xy)
+    xb = bytearray(32 * n)
+    yb = bytearray(32 * n)
+    ident = np.zeros(n, dtype=bool)
+    for i, p in enumerate(points_xy):
+        if p is None:
+            ident[i] = True
+            continue
+        xb[32 * i:32 * i + 32] = p[0].to_bytes(32, "little")
+        yb[32 * i:32 * i + 32] = p[1].to_bytes(32, "little")
+    return jacobian_from_affine_bytes(bytes(xb), bytes(yb), ident)
+
+
+def jacobian_from_affine_bytes(xb, yb, ident=None):
+    """Affine coordinates as n x 32-byte little-endian strings -> reference Jacobian structs."""
+    xw = np.frombuffer(xb, dtype="<u8").reshape(-1, 4)
+    yw = np.frombuffer(yb, dtype="<u8").reshape(-1, 4)
+    n = xw.shape[0]
+    out = np.zeros((n, 15), dtype=np.uint64)
+    out[:, 0:5] = _words_to_r51(xw)
+    out[:, 5:10] = _words_to_r51(yw)
+    out[:, 10] = 1
+    if ident is not None and ident.any():
+        out[ident] = 0
+        out[ident, 0] = 1
+        out[ident, 5] = 1
+    return out
+
+
+def scalars_array(scalars):
+    """ints or 32-byte strings -> uint8 array (n, 32) little-endian."""
+    buf = bytearray()
+    for s in scalars:
+        buf += s if isinstance(s, (bytes, bytearray)) else int(s).to_bytes(32, "little")
+    return np.frombuffer(bytes(buf), dtype=np.uint8).reshape(-1, 32).copy()
+
+
+# ---------------------------------------------------------------- MSM calls
+def msm(curve, scalars, points, flags=0):
+    """Plain MSM through the status-returning slot.  scalars: uint8 (n, 32); points: uint64
+    (n, 15) reference Jacobian structs.  Returns (compressed 32 bytes, jacobian uint64[15])."""
+    c = CURVES[curve]
+    s = np.ascontiguousarray(scalars, dtype=np.uint8)
+    p = np.ascontiguousarray(points, dtype=np.uint64)
+    n = p.shape[0]
+    assert s.shape == (n, 32) and p.shape == (n, 15)
+    out = np.zeros(32, dtype=np.uint8)
+    jac = np.zeros(15, dtype=np.uint64)
+    _check(lib().hs_gpu_msm_ex(c, _ptr(jac), _ptr(out), _ptr(s), _ptr(p), n, flags))
+    return out.tobytes(), jac
+
+
+def msm_slot(curve, scalars, points):
+    """The void drop-in slot (hs_gpu_helios_msm_vartime / hs_gpu_selene_msm_vartime)."""
+    c = CURVES[curve]
+    s = np.ascontiguousarray(scalars, dtype=np.uint8)
+    p = np.ascontiguousarray(points, dtype=np.uint64)
+    jac = np.zeros(15, dtype=np.uint64)
+    f = lib().hs_gpu_helios_msm_vartime if c == HELIOS else lib().hs_gpu_selene_msm_vartime
+    f(_ptr(jac), _ptr(s), _ptr(p), p.shape[0])
+    return jac
+
+
+def msm_device(curve, d_scalars, d_points, n, d_out32, d_out_jac=None, flags=0, stream=None):
+    """Device-resident MSM: arguments are device addresses (ints, e.g. torch tensor data_ptr())."""
+    _check(lib().hs_gpu_msm_device(CURVES[curve], d_scalars, d_points, n, flags, d_out32, d_out_jac, stream))
+
+
+def synchronize():
+    _check(lib().hs_gpu_synchronize())
+
+
+def window_bits(curve, n):
+    return lib().hs_gpu_window_bits(CURVES[curve], n)
+
+
+def reference_window_bits(n):
+    return lib().hs_gpu_reference_window_bits(n)

@@ -1,0 +1,342 @@
+// Host side of the MI355X MSM library: per-device context, scratch management, launch sequence of
+// the plain Pippenger path, and the C ABI declared in include/hs_gpu_msm.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hs_gpu_msm.h"
+#include "msm_kernels.h"
+
+namespace
+{
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HS_CHECK(expr)                                                                                        \
+    do                                                                                                        \
+    {                                                                                                         \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return fail(HS_GPU_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// A growable device buffer.
+struct DevBuf
+{
+    void *p = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes)
+    {
+        if (bytes <= cap)
+            return 0;
+        if (p)
+            hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes + bytes / 4 + 4096;
+        if (hipMalloc(&p, want) != hipSuccess)
+        {
+            p = nullptr;
+            return fail(HS_GPU_ENOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+        }
+        cap = want;
+        return 0;
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct Plan
+{
+    int c, W;
+    uint32_t B, G, nb, segs;
+};
+
+// Window width for an n-point MSM: balances W*n bucket additions against the W*2^c reduction
+// work; c * W >= 256 always holds since W = ceil(256 / c).
+int choose_c(size_t n)
+{
+    int lg = 0;
+    while (((size_t)1 << (lg + 1)) <= n)
+        lg++;
+    int c = lg - 4;
+    if (c < 4)
+        c = 4;
+    if (c > 16)
+        c = 16;
+    return c;
+}
+
+Plan make_plan(size_t n)
+{
+    Plan p;
+    p.c = choose_c(n);
+    p.W = (256 + p.c - 1) / p.c;
+    p.B = 1u << (p.c - 1);
+    p.G = p.B < 16 ? p.B : 16;
+    p.nb = (uint32_t)p.W * p.B;
+    p.segs = p.nb / p.G;
+    return p;
+}
+
+int ref_window(size_t n)
+{
+    if (n <= 16)
+        return 0;
+    if (n < 96)
+        return 5;
+    if (n < 288)
+        return 6;
+    if (n < 864)
+        return 7;
+    if (n < 2592)
+        return 8;
+    if (n < 7776)
+        return 9;
+    if (n < 23328)
+        return 10;
+    return 11;
+}
+
+struct Ctx
+{
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevBuf scalars, points, aff, isid, digits, counts, offsets, block_sums, sorted, buckets, seg, win, big_list,
+        big_count, out32, outjac;
+    bool ready = false;
+
+    int init()
+    {
+        if (ready)
+            return 0;
+        int dev = 0;
+        HS_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HS_CHECK(hipGetDeviceProperties(&prop, dev));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return fail(HS_GPU_EDEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+        device = dev;
+        HS_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        ready = true;
+        return 0;
+    }
+};
+
+Ctx &ctx()
+{
+    static Ctx c;
+    return c;
+}
+
+int scan_exclusive(Ctx &cx, const uint32_t *in, uint32_t *out, uint32_t len, hipStream_t st)
+{
+    const uint32_t per = hs::SCAN_BLOCK * hs::SCAN_ITEMS;
+    const uint32_t nblk = (len + per - 1) / per;
+    if (nblk > per)
+        return fail(HS_GPU_EINVAL, "scan too long");
+    if (int r = cx.block_sums.reserve(2 * (size_t)(nblk + 1) * 4 + 64))
+        return r;
+    uint32_t *bs = cx.block_sums.as<uint32_t>();
+    uint32_t *bs_off = bs + nblk + 1;
+    hipLaunchKernelGGL(hs::k_scan_local, dim3(nblk), dim3(hs::SCAN_BLOCK), 0, st, in, out, len, bs);
+    uint32_t *dummy = bs_off + nblk + 1;
+    hipLaunchKernelGGL(hs::k_scan_local, dim3(1), dim3(hs::SCAN_BLOCK), 0, st, bs, bs_off, nblk, dummy);
+    hipLaunchKernelGGL(hs::k_scan_add, dim3(nblk), dim3(hs::SCAN_BLOCK), 0, st, out, len, bs_off);
+    HS_CHECK(hipGetLastError());
+    return 0;
+}
+
+// The plain MSM on device-resident inputs.  d_scalars: n x 8 words; d_points: n x 15 uint64.
+template <class C>
+int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, size_t n, unsigned flags, uint32_t *d_out32,
+               uint64_t *d_outjac, hipStream_t st)
+{
+    if (n >= (1ull << 31))
+        return fail(HS_GPU_EINVAL, "n too large");
+    const Plan P = make_plan(n);
+    const uint32_t nn = (uint32_t)n;
+    const bool compat = !(flags & HS_GPU_EXACT) && ref_window(n) == 8;
+    int r;
+    if ((r = cx.aff.reserve(n * 64)) || (r = cx.isid.reserve(n)) || (r = cx.digits.reserve(n * 4 * (size_t)P.W)) ||
+        (r = cx.counts.reserve((size_t)(P.nb + 1) * 4)) || (r = cx.offsets.reserve((size_t)(P.nb + 1) * 4)) ||
+        (r = cx.sorted.reserve(n * 4 * (size_t)P.W + 4)) || (r = cx.buckets.reserve((size_t)P.nb * sizeof(hs::xyzz))) ||
+        (r = cx.seg.reserve((size_t)P.segs * sizeof(hs::xyzz))) || (r = cx.win.reserve((size_t)P.W * sizeof(hs::xyzz))) ||
+        (r = cx.big_list.reserve((size_t)P.nb * 4)) || (r = cx.big_count.reserve(64)))
+        return r;
+    hs::aff *aff = cx.aff.as<hs::aff>();
+    uint8_t *isid = cx.isid.as<uint8_t>();
+    uint32_t *digits = cx.digits.as<uint32_t>(), *counts = cx.counts.as<uint32_t>(),
+             *offsets = cx.offsets.as<uint32_t>(), *sorted = cx.sorted.as<uint32_t>();
+    hs::xyzz *buckets = cx.buckets.as<hs::xyzz>(), *seg = cx.seg.as<hs::xyzz>(), *win = cx.win.as<hs::xyzz>();
+    uint32_t *big_list = cx.big_list.as<uint32_t>(), *big_count = cx.big_count.as<uint32_t>();
+
+    const dim3 blk(256), grd((nn + 255) / 256);
+    hipLaunchKernelGGL(hs::k_ingest<C>, grd, blk, 0, st, d_points, nn, aff, isid);
+    HS_CHECK(hipMemsetAsync(counts, 0, (size_t)(P.nb + 1) * 4, st));
+    HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
+    hipLaunchKernelGGL(hs::k_recode<C>, grd, blk, 0, st, d_scalars, isid, nn, P.c, P.W, compat ? 1 : 0, digits, counts);
+    if ((r = scan_exclusive(cx, counts, offsets, P.nb + 1, st)))
+        return r;
+    // cursor = offsets (counts reused as the cursor array)
+    HS_CHECK(hipMemcpyAsync(counts, offsets, (size_t)P.nb * 4, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(hs::k_scatter, grd, blk, 0, st, digits, nn, P.W, P.B, counts, sorted);
+    const uint32_t lmax = 96;
+    hipLaunchKernelGGL(hs::k_bucket_small<C>, dim3((P.nb + 255) / 256), blk, 0, st, offsets, P.nb, sorted, aff, lmax,
+                       buckets, big_list, big_count);
+    hipLaunchKernelGGL(hs::k_bucket_big<C>, dim3(512), dim3(hs::BIG_THREADS), 0, st, offsets, sorted, aff, big_list,
+                       big_count, buckets);
+    hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((P.segs + 255) / 256), blk, 0, st, buckets, P.B, P.G, P.segs, seg);
+    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(P.W), dim3(hs::BIG_THREADS), 0, st, seg, P.segs / P.W, win);
+    hipLaunchKernelGGL(hs::k_combine<C>, dim3(1), dim3(64), 0, st, win, P.W, P.c, d_out32, d_outjac);
+    HS_CHECK(hipGetLastError());
+    return 0;
+}
+
+int msm_device_any(int curve, Ctx &cx, const void *s, const void *p, size_t n, unsigned flags, void *o32, void *oj,
+                   hipStream_t st)
+{
+    if (curve == HS_GPU_HELIOS)
+        return msm_device<hs::HeliosC>(cx, (const uint32_t *)s, (const uint64_t *)p, n, flags, (uint32_t *)o32,
+                                       (uint64_t *)oj, st);
+    if (curve == HS_GPU_SELENE)
+        return msm_device<hs::SeleneC>(cx, (const uint32_t *)s, (const uint64_t *)p, n, flags, (uint32_t *)o32,
+                                       (uint64_t *)oj, st);
+    return fail(HS_GPU_EINVAL, "unknown curve id");
+}
+
+void set_identity(hs_gpu_jacobian *r)
+{
+    std::memset(r, 0, sizeof *r);
+    r->X[0] = 1;
+    r->Y[0] = 1;
+}
+
+int msm_host(int curve, hs_gpu_jacobian *result, unsigned char *out32, const unsigned char *scalars,
+             const hs_gpu_jacobian *points, size_t n, unsigned flags)
+{
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        return fail(HS_GPU_EINVAL, "unknown curve id");
+    if (n == 0)
+    {
+        if (result)
+            set_identity(result);
+        if (out32)
+            std::memset(out32, 0, 32);
+        return 0;
+    }
+    if (!scalars || !points)
+        return fail(HS_GPU_EINVAL, "null input with n > 0");
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()))
+        return r;
+    if ((r = cx.scalars.reserve(n * 32)) || (r = cx.points.reserve(n * sizeof(hs_gpu_jacobian))) ||
+        (r = cx.out32.reserve(64)) || (r = cx.outjac.reserve(sizeof(hs_gpu_jacobian))))
+        return r;
+    hipStream_t st = cx.stream;
+    HS_CHECK(hipMemcpyAsync(cx.scalars.p, scalars, n * 32, hipMemcpyHostToDevice, st));
+    HS_CHECK(hipMemcpyAsync(cx.points.p, points, n * sizeof(hs_gpu_jacobian), hipMemcpyHostToDevice, st));
+    if ((r = msm_device_any(curve, cx, cx.scalars.p, cx.points.p, n, flags, cx.out32.p, cx.outjac.p, st)))
+        return r;
+    unsigned char o32[32];
+    hs_gpu_jacobian oj;
+    HS_CHECK(hipMemcpyAsync(o32, cx.out32.p, 32, hipMemcpyDeviceToHost, st));
+    HS_CHECK(hipMemcpyAsync(&oj, cx.outjac.p, sizeof oj, hipMemcpyDeviceToHost, st));
+    HS_CHECK(hipStreamSynchronize(st));
+    if (out32)
+        std::memcpy(out32, o32, 32);
+    if (result)
+        std::memcpy(result, &oj, sizeof oj);
+    return 0;
+}
+
+void slot(int curve, hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_jacobian *points, size_t n)
+{
+    int r = msm_host(curve, result, nullptr, scalars, points, n, 0);
+    if (r != 0)
+    {
+        std::fprintf(stderr, "hs_gpu %s_msm_vartime failed (%d): %s\n", curve == 0 ? "helios" : "selene", r,
+                     g_err.c_str());
+        if (result)
+            set_identity(result);
+    }
+}
+
+} // namespace
+
+extern "C"
+{
+
+void hs_gpu_helios_msm_vartime(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_jacobian *points,
+                               size_t n)
+{
+    slot(HS_GPU_HELIOS, result, scalars, points, n);
+}
+
+void hs_gpu_selene_msm_vartime(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_jacobian *points,
+                               size_t n)
+{
+    slot(HS_GPU_SELENE, result, scalars, points, n);
+}
+
+int hs_gpu_msm_ex(int curve, hs_gpu_jacobian *result, unsigned char out32[32], const unsigned char *scalars,
+                  const hs_gpu_jacobian *points, size_t n, unsigned flags)
+{
+    return msm_host(curve, result, out32, scalars, points, n, flags);
+}
+
+int hs_gpu_msm_device(int curve, const void *d_scalars, const void *d_points, size_t n, unsigned flags, void *d_out32,
+                      void *d_out_jac, void *stream)
+{
+    if (n == 0 || !d_scalars || !d_points || !d_out32)
+        return fail(HS_GPU_EINVAL, "hs_gpu_msm_device needs n > 0 and device buffers");
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()))
+        return r;
+    hipStream_t st = stream ? (hipStream_t)stream : cx.stream;
+    return msm_device_any(curve, cx, d_scalars, d_points, n, flags, d_out32, d_out_jac, st);
+}
+
+int hs_gpu_synchronize(void)
+{
+    Ctx &cx = ctx();
+    if (!cx.ready)
+        return 0;
+    HS_CHECK(hipStreamSynchronize(cx.stream));
+    return 0;
+}
+
+int hs_gpu_init(void)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    return cx.init();
+}
+
+const char *hs_gpu_last_error(void) { return g_err.c_str(); }
+
+const char *hs_gpu_version(void) { return "hs_gpu 0.1 gfx950"; }
+
+int hs_gpu_window_bits(int curve, size_t n)
+{
+    (void)curve;
+    return choose_c(n);
+}
+
+int hs_gpu_reference_window_bits(size_t n) { return ref_window(n); }
+
+} // extern "C"

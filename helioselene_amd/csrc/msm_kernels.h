@@ -1,0 +1,469 @@
+// Pippenger MSM kernels for gfx950 (one plain MSM per launch sequence).
+//
+// Pipeline for  Q = sum s_i P_i  (reference helios/src/x64/helios_msm_vartime.cpp:265-388):
+//   ingest   radix-2^51 Jacobian points (the reference struct) -> canonical affine, identity flags
+//   recode   scalars -> signed width-c digits for W = ceil(256/c) windows, bucket histogram
+//   sort     per-window counting sort of (window, bucket) keys: scan of the histogram + scatter
+//   bucket   per-bucket sums in xyzz with mixed additions (thread per bucket; one workgroup per
+//            oversized bucket with an LDS tree)
+//   reduce   per-window running sums over segments of buckets, then a tree over segments
+//   combine  Horner over windows (c doublings per window), affine + 32-byte encoding
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "curve.h"
+#include "scalar.h"
+
+namespace hs
+{
+
+struct HeliosC
+{
+    using F = Fp; // coordinate field
+    using O = Fq; // scalar field (group order q)
+    static constexpr int id = 0;
+};
+
+struct SeleneC
+{
+    using F = Fq;
+    using O = Fp; // group order p
+    static constexpr int id = 1;
+};
+
+// ---------------------------------------------------------------- memory helpers
+__device__ __forceinline__ void load_aff(aff &a, const aff *base, uint32_t idx)
+{
+    const uint4 *p = reinterpret_cast<const uint4 *>(base + idx);
+    uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+    a.x.v[0] = q0.x; a.x.v[1] = q0.y; a.x.v[2] = q0.z; a.x.v[3] = q0.w;
+    a.x.v[4] = q1.x; a.x.v[5] = q1.y; a.x.v[6] = q1.z; a.x.v[7] = q1.w;
+    a.y.v[0] = q2.x; a.y.v[1] = q2.y; a.y.v[2] = q2.z; a.y.v[3] = q2.w;
+    a.y.v[4] = q3.x; a.y.v[5] = q3.y; a.y.v[6] = q3.z; a.y.v[7] = q3.w;
+}
+
+__device__ __forceinline__ void store_fe(uint32_t *p, const fe &a)
+{
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    q[0] = make_uint4(a.v[0], a.v[1], a.v[2], a.v[3]);
+    q[1] = make_uint4(a.v[4], a.v[5], a.v[6], a.v[7]);
+}
+
+__device__ __forceinline__ void load_fe(fe &a, const uint32_t *p)
+{
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    uint4 q0 = q[0], q1 = q[1];
+    a.v[0] = q0.x; a.v[1] = q0.y; a.v[2] = q0.z; a.v[3] = q0.w;
+    a.v[4] = q1.x; a.v[5] = q1.y; a.v[6] = q1.z; a.v[7] = q1.w;
+}
+
+__device__ __forceinline__ void store_xyzz(xyzz *dst, const xyzz &p)
+{
+    uint32_t *b = reinterpret_cast<uint32_t *>(dst);
+    store_fe(b, p.X);
+    store_fe(b + 8, p.Y);
+    store_fe(b + 16, p.ZZ);
+    store_fe(b + 24, p.ZZZ);
+}
+
+__device__ __forceinline__ void load_xyzz(xyzz &p, const xyzz *src)
+{
+    const uint32_t *b = reinterpret_cast<const uint32_t *>(src);
+    load_fe(p.X, b);
+    load_fe(p.Y, b + 8);
+    load_fe(p.ZZ, b + 16);
+    load_fe(p.ZZZ, b + 24);
+}
+
+// ---------------------------------------------------------------- ingest
+// Reference points: helios_jacobian = {X[5], Y[5], Z[5]} radix-2^51 uint64 limbs, Z = 0 is the
+// identity (helios/include/helios_ops.h:44-64).  Output canonical affine (X/Z^2, Y/Z^3).
+template <class C>
+__global__ void __launch_bounds__(256) k_ingest(const uint64_t *__restrict__ jac, uint32_t n, aff *__restrict__ out,
+                                                uint8_t *__restrict__ is_id)
+{
+    using F = typename C::F;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t *p = jac + 15ull * i;
+    uint64_t l[5];
+    fe X, Y, Z;
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        l[k] = p[k];
+    fe_from_r51<F>(X, l);
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        l[k] = p[5 + k];
+    fe_from_r51<F>(Y, l);
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        l[k] = p[10 + k];
+    fe_from_r51<F>(Z, l);
+    aff a;
+    fe Zc;
+    fe_canon<F>(Zc, Z);
+    uint32_t zor = 0, one = Zc.v[0] == 1u;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        zor |= Zc.v[k];
+#pragma unroll
+    for (int k = 1; k < 8; k++)
+        one &= Zc.v[k] == 0u;
+    if (zor == 0)
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+        is_id[i] = 1;
+    }
+    else
+    {
+        if (one)
+        {
+            fe_canon<F>(a.x, X);
+            fe_canon<F>(a.y, Y);
+        }
+        else
+        {
+            fe zi, zi2, zi3, t;
+            fe_inv<F>(zi, Z);
+            fe_sq<F>(zi2, zi);
+            fe_mul<F>(zi3, zi2, zi);
+            fe_mul<F>(t, X, zi2);
+            fe_canon<F>(a.x, t);
+            fe_mul<F>(t, Y, zi3);
+            fe_canon<F>(a.y, t);
+        }
+        is_id[i] = 0;
+    }
+    uint32_t *o = reinterpret_cast<uint32_t *>(out + i);
+    store_fe(o, a.x);
+    store_fe(o + 8, a.y);
+}
+
+// ---------------------------------------------------------------- recode + histogram
+// digits[w * n + i]: 0, or |d| | sign << 31;  counts[w * B + |d| - 1] += 1
+template <class C>
+__global__ void __launch_bounds__(256) k_recode(const uint32_t *__restrict__ scalars, const uint8_t *__restrict__ is_id,
+                                                uint32_t n, int c, int W, int w8_compat,
+                                                uint32_t *__restrict__ digits, uint32_t *__restrict__ counts)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t s[8];
+    const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * i);
+    uint4 a = sp[0], b = sp[1];
+    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+    s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    fe k;
+    scalar_prepare<typename C::O>(k, s, w8_compat != 0);
+    const bool skip = is_id[i] != 0;
+    const uint32_t B = 1u << (c - 1);
+    uint32_t carry = 0;
+    for (int w = 0; w < W; w++)
+    {
+        uint32_t d = recode_digit(k, c, w, carry);
+        if (skip)
+            d = 0;
+        digits[(size_t)w * n + i] = d;
+        if (d)
+            atomicAdd(&counts[(size_t)w * B + (d & 0x7fffffffu) - 1], 1u);
+    }
+}
+
+// ---------------------------------------------------------------- exclusive scan (uint32)
+// Three-phase scan: per-block scan of 2048 elements -> block sums -> add offsets.
+constexpr int SCAN_BLOCK = 1024, SCAN_ITEMS = 2;
+
+// Exclusive scan of one value per thread across the block (blockDim.x a multiple of 64, <= 1024).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v)
+{
+    __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[wid] = x;
+    __syncthreads();
+    if (wid == 0)
+    {
+        uint32_t t = lane < nw ? wsum[lane] : 0u;
+        uint32_t u = t;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            uint32_t y = __shfl_up(u, o, 64);
+            if (lane >= o)
+                u += y;
+        }
+        if (lane < nw)
+            wsum[lane] = u - t;
+    }
+    __syncthreads();
+    uint32_t r = x - v + wsum[wid];
+    __syncthreads(); // wsum is reused by the next call
+    return r;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_local(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                           uint32_t len, uint32_t *__restrict__ block_sums)
+{
+    __shared__ uint32_t tot;
+    const uint32_t base = blockIdx.x * (SCAN_BLOCK * SCAN_ITEMS) + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS], s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+    {
+        v[k] = base + k < len ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t ex = block_exclusive_scan(s);
+    if (threadIdx.x == blockDim.x - 1)
+        tot = ex + s;
+    __syncthreads();
+    uint32_t run = ex;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+    {
+        if (base + k < len)
+            out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0)
+        block_sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(uint32_t *__restrict__ out, uint32_t len,
+                                                         const uint32_t *__restrict__ block_offsets)
+{
+    const uint32_t base = blockIdx.x * (SCAN_BLOCK * SCAN_ITEMS) + threadIdx.x * SCAN_ITEMS;
+    const uint32_t off = block_offsets[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < len)
+            out[base + k] += off;
+}
+
+// ---------------------------------------------------------------- scatter into sorted order
+__global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ digits, uint32_t n, int W, uint32_t B,
+                                                 uint32_t *__restrict__ cursor, uint32_t *__restrict__ sorted)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int w = 0; w < W; w++)
+    {
+        uint32_t d = digits[(size_t)w * n + i];
+        if (d)
+        {
+            uint32_t pos = atomicAdd(&cursor[(size_t)w * B + (d & 0x7fffffffu) - 1], 1u);
+            sorted[pos] = i | (d & 0x80000000u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- bucket sums
+template <class C>
+__device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const aff *__restrict__ pts, uint32_t e)
+{
+    using F = typename C::F;
+    aff p;
+    load_aff(p, pts, e & 0x7fffffffu);
+    if (e >> 31)
+        fe_neg<F>(p.y, p.y);
+    xyzz_madd<F>(acc, acc_id, p);
+}
+
+// one thread per bucket with at most `lmax` entries; larger buckets are queued for k_bucket_big
+template <class C>
+__global__ void __launch_bounds__(256) k_bucket_small(const uint32_t *__restrict__ offsets, uint32_t nbuckets,
+                                                      const uint32_t *__restrict__ sorted, const aff *__restrict__ pts,
+                                                      uint32_t lmax, xyzz *__restrict__ buckets,
+                                                      uint32_t *__restrict__ big_list, uint32_t *__restrict__ big_count)
+{
+    using F = typename C::F;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nbuckets)
+        return;
+    const uint32_t beg = offsets[g], end = offsets[g + 1], cnt = end - beg;
+    if (cnt > lmax)
+    {
+        uint32_t k = atomicAdd(big_count, 1u);
+        big_list[k] = g;
+        return;
+    }
+    xyzz acc;
+    bool acc_id = true;
+    xyzz_identity<F>(acc);
+    for (uint32_t j = beg; j < end; j++)
+        add_entry<C>(acc, acc_id, pts, sorted[j]);
+    if (acc_id)
+        xyzz_identity<F>(acc);
+    store_xyzz(buckets + g, acc);
+}
+
+constexpr int BIG_THREADS = 256;
+
+// LDS tree sum of one xyzz per thread; result in thread 0.  sh holds BIG_THREADS points.
+template <class C> __device__ void block_tree_sum(xyzz &acc, xyzz *sh)
+{
+    using F = typename C::F;
+    const int t = threadIdx.x;
+    store_xyzz(sh + t, acc);
+    __syncthreads();
+    for (int s = blockDim.x >> 1; s > 0; s >>= 1)
+    {
+        if (t < s)
+        {
+            xyzz a, b;
+            load_xyzz(a, sh + t);
+            load_xyzz(b, sh + t + s);
+            xyzz_add<F>(a, a, b);
+            store_xyzz(sh + t, a);
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        load_xyzz(acc, sh);
+}
+
+// one workgroup per oversized bucket: strided per-thread sums, then an LDS tree
+template <class C>
+__global__ void __launch_bounds__(BIG_THREADS) k_bucket_big(const uint32_t *__restrict__ offsets,
+                                                            const uint32_t *__restrict__ sorted,
+                                                            const aff *__restrict__ pts,
+                                                            const uint32_t *__restrict__ big_list,
+                                                            const uint32_t *__restrict__ big_count,
+                                                            xyzz *__restrict__ buckets)
+{
+    using F = typename C::F;
+    __shared__ xyzz sh[BIG_THREADS];
+    const uint32_t nbig = *big_count;
+    for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x)
+    {
+        const uint32_t g = big_list[k];
+        const uint32_t beg = offsets[g], end = offsets[g + 1];
+        xyzz acc;
+        bool acc_id = true;
+        xyzz_identity<F>(acc);
+        for (uint32_t j = beg + threadIdx.x; j < end; j += blockDim.x)
+            add_entry<C>(acc, acc_id, pts, sorted[j]);
+        if (acc_id)
+            xyzz_identity<F>(acc);
+        block_tree_sum<C>(acc, sh);
+        if (threadIdx.x == 0)
+            store_xyzz(buckets + g, acc);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- bucket reduction
+// Window w, segment s covers buckets [s*G, s*G + G) (bucket b holds digit magnitude b + 1).
+// Running sums give T = sum (b - s*G + 1) B_b and R = sum B_b; the segment's share of
+// sum (b+1) B_b is T + (s*G) R (reference running-sum reduction, helios_msm_vartime.cpp:334-361).
+template <class C>
+__global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buckets, uint32_t B, uint32_t G,
+                                                    uint32_t nseg_total, xyzz *__restrict__ seg_out)
+{
+    using F = typename C::F;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nseg_total)
+        return;
+    const uint32_t segs_per_win = B / G;
+    const uint32_t w = t / segs_per_win, s = t % segs_per_win;
+    const xyzz *bk = buckets + (size_t)w * B + (size_t)s * G;
+    xyzz R, T;
+    xyzz_identity<F>(R);
+    xyzz_identity<F>(T);
+    for (int j = (int)G - 1; j >= 0; j--)
+    {
+        xyzz b;
+        load_xyzz(b, bk + j);
+        xyzz_add<F>(R, R, b);
+        xyzz_add<F>(T, T, R);
+    }
+    xyzz U;
+    xyzz_mul_small<F>(U, R, s * G);
+    xyzz_add<F>(T, T, U);
+    store_xyzz(seg_out + t, T);
+}
+
+// one workgroup per window: sum of the window's segment results
+template <class C>
+__global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win,
+                                                            xyzz *__restrict__ win_out)
+{
+    using F = typename C::F;
+    __shared__ xyzz sh[BIG_THREADS];
+    const uint32_t w = blockIdx.x;
+    xyzz acc;
+    xyzz_identity<F>(acc);
+    for (uint32_t s = threadIdx.x; s < segs_per_win; s += blockDim.x)
+    {
+        xyzz b;
+        load_xyzz(b, seg_out + (size_t)w * segs_per_win + s);
+        xyzz_add<F>(acc, acc, b);
+    }
+    block_tree_sum<C>(acc, sh);
+    if (threadIdx.x == 0)
+        store_xyzz(win_out + w, acc);
+}
+
+// ---------------------------------------------------------------- window combine + encode
+// out32: compressed point (8 words); out_jac: reference Jacobian struct (15 uint64): (x : y : 1)
+// canonical, or (1 : 1 : 0) for the identity (helios_ops.h:44-50).
+template <class C>
+__global__ void k_combine(const xyzz *__restrict__ win_out, int W, int c, uint32_t *__restrict__ out32,
+                          uint64_t *__restrict__ out_jac)
+{
+    using F = typename C::F;
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    xyzz acc;
+    load_xyzz(acc, win_out + (W - 1));
+    for (int w = W - 2; w >= 0; w--)
+    {
+        for (int k = 0; k < c; k++)
+            xyzz_dbl<F>(acc, acc);
+        xyzz b;
+        load_xyzz(b, win_out + w);
+        xyzz_add<F>(acc, acc, b);
+    }
+    aff a;
+    bool pt = xyzz_to_aff<F>(a, acc);
+    uint32_t enc[8];
+    encode_aff(enc, a, pt);
+    if (out32)
+        for (int k = 0; k < 8; k++)
+            out32[k] = enc[k];
+    if (out_jac)
+    {
+        uint64_t l[5];
+        if (pt)
+        {
+            fe_to_r51<F>(l, a.x);
+            for (int k = 0; k < 5; k++)
+                out_jac[k] = l[k];
+            fe_to_r51<F>(l, a.y);
+            for (int k = 0; k < 5; k++)
+                out_jac[5 + k] = l[k];
+            for (int k = 0; k < 5; k++)
+                out_jac[10 + k] = k == 0 ? 1u : 0u;
+        }
+        else
+        {
+            for (int k = 0; k < 15; k++)
+                out_jac[k] = (k == 0 || k == 5) ? 1u : 0u;
+        }
+    }
+}
+
+} // namespace hs

@@ -1,0 +1,98 @@
+/*
+ * hs_gpu_msm.h -- C ABI of the MI355X (gfx950) variable-time multi-scalar multiplication for the
+ * Helios / Selene curve cycle.  Plain pointers and sizes only; every entry point is callable from
+ * C, C++ (the reference's HeliosPoint / SelenePoint layer), ctypes or any FFI.
+ *
+ * Curves: HS_GPU_HELIOS (coordinates in F_p, p = 2^255-19, group order q),
+ *         HS_GPU_SELENE (coordinates in F_q, q = 2^255-gamma, group order p).
+ *
+ * Formats (identical to the reference):
+ *   scalars  n x 32 bytes little-endian, contiguous (reference api_point.cpp:103-109); any
+ *            256-bit value is accepted.
+ *   points   hs_gpu_jacobian[n]: the reference's helios_jacobian / selene_jacobian struct,
+ *            X, Y, Z as uint64_t[5] radix-2^51 limbs (lazy limbs allowed), affine
+ *            (X/Z^2, Y/Z^3), Z = 0 is the identity (helios/include/helios.h:41-46,
+ *            helios_ops.h:44-64).
+ *   out32    32-byte compressed point: canonical x little-endian, bit 255 = parity of canonical
+ *            y, identity = 32 zero bytes (helios/src/x64/helios_tobytes.cpp:42-66).
+ *
+ * Result semantics: by default the result equals the reference CPU MSM byte for byte, including
+ * the reference's behaviour for 864 <= n < 2592 (its window-8 recoding drops a final carry,
+ * helios_msm_vartime.cpp:127-162 with :256-257).  HS_GPU_EXACT computes sum s_i * P_i exactly.
+ *
+ * Errors: functions returning int give 0 on success or a negative HS_GPU_E* code;
+ * hs_gpu_last_error() describes the most recent failure of the calling thread.
+ * Threading: every entry point may be called from several host threads; calls on one device are
+ * serialised internally (one in-flight plain MSM per device context).
+ */
+#ifndef HS_GPU_MSM_H
+#define HS_GPU_MSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HS_GPU_HELIOS 0
+#define HS_GPU_SELENE 1
+
+/* flags */
+#define HS_GPU_EXACT 1u /* exact sum; do not reproduce the reference's window-8 carry drop */
+
+/* status codes */
+#define HS_GPU_OK 0
+#define HS_GPU_EINVAL (-1)   /* bad argument (curve id, null pointer with n > 0, size) */
+#define HS_GPU_EDEVICE (-2)  /* HIP runtime / kernel failure, or no gfx950 device */
+#define HS_GPU_ENOMEM (-3)   /* device allocation failed */
+#define HS_GPU_EDECODE (-4)  /* a compressed point failed to decode */
+
+/* Layout-identical to the reference's helios_jacobian and selene_jacobian (120 bytes). */
+typedef struct hs_gpu_jacobian
+{
+    uint64_t X[5];
+    uint64_t Y[5];
+    uint64_t Z[5];
+} hs_gpu_jacobian;
+
+/* ---------------------------------------------------------------- drop-in slot replacements
+ * Same signature and semantics as the reference's dispatch slots
+ *   void helios_msm_vartime(helios_jacobian *result, const unsigned char *scalars,
+ *                           const helios_jacobian *points, size_t n)
+ * (helios/include/helios_msm_vartime.h:43-72; slot types include/helioselene_dispatch.h:53,56;
+ * backends helios_msm_vartime_{x64,avx2,ifma}, src/dispatch.cpp:133-179) and the selene_ twin.
+ * n == 0 gives the identity (helios_msm_vartime.cpp:402-406).  The result is written last, so it
+ * may alias the inputs.  On a device failure the message is printed to stderr, the result is set
+ * to the identity and hs_gpu_last_error() reports the cause (use hs_gpu_msm_ex for a status). */
+void hs_gpu_helios_msm_vartime(hs_gpu_jacobian *result, const unsigned char *scalars,
+                               const hs_gpu_jacobian *points, size_t n);
+void hs_gpu_selene_msm_vartime(hs_gpu_jacobian *result, const unsigned char *scalars,
+                               const hs_gpu_jacobian *points, size_t n);
+
+/* Status-returning form of the slot, with flags.  Either output may be NULL. */
+int hs_gpu_msm_ex(int curve, hs_gpu_jacobian *result, unsigned char out32[32], const unsigned char *scalars,
+                  const hs_gpu_jacobian *points, size_t n, unsigned flags);
+
+/* Device-resident form: all pointers are device pointers (HBM), the call is asynchronous on
+ * `stream` (a hipStream_t, NULL = the library's stream for this device).  d_out32 receives the
+ * 32-byte encoding, d_out_jac (may be NULL) the reference struct. */
+int hs_gpu_msm_device(int curve, const void *d_scalars, const void *d_points, size_t n, unsigned flags,
+                      void *d_out32, void *d_out_jac, void *stream);
+
+/* Waits for all work the library queued on its own stream(s). */
+int hs_gpu_synchronize(void);
+
+/* ---------------------------------------------------------------- library */
+int hs_gpu_init(void);              /* selects device 0 (or the current device); idempotent */
+const char *hs_gpu_last_error(void);
+const char *hs_gpu_version(void);
+/* Window width the library uses for an n-point MSM on `curve`. */
+int hs_gpu_window_bits(int curve, size_t n);
+/* Window width the reference CPU MSM uses (helios_msm_vartime.cpp:248-263), 0 for n <= 16. */
+int hs_gpu_reference_window_bits(size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
