@@ -49,6 +49,14 @@ def lib():
         L.hs_gpu_version.restype = cp
         L.hs_gpu_window_bits.argtypes = [ctypes.c_int, sz]
         L.hs_gpu_reference_window_bits.argtypes = [sz]
+        L.hs_gpu_msm_bytes.argtypes = [ctypes.c_int, vp, vp, vp, sz, ctypes.c_uint]
+        L.hs_gpu_map_to_curve.argtypes = [ctypes.c_int, vp, sz, vp, vp]
+        L.hs_gpu_map_to_curve_device.argtypes = [ctypes.c_int, vp, sz, vp, vp, vp]
+        L.hs_gpu_profile.argtypes = [ctypes.c_int]
+        L.hs_gpu_phase_times.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+        L.hs_gpu_phase_name.restype = cp
+        L.hs_gpu_probe_int_mul_peak.argtypes = [ctypes.POINTER(ctypes.c_double)]
+        L.hs_gpu_debug_dump.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
 
@@ -162,3 +170,59 @@ def window_bits(curve, n):
 
 def reference_window_bits(n):
     return lib().hs_gpu_reference_window_bits(n)
+
+
+def msm_bytes(curve, scalars, points32, flags=0):
+    """MSM over compressed points (decoded on the device).  scalars: bytes/uint8 (n*32);
+    points32: bytes (n*32).  Returns the 32-byte result."""
+    s = np.frombuffer(bytes(scalars), dtype=np.uint8)
+    p = np.frombuffer(bytes(points32), dtype=np.uint8)
+    n = p.size // 32
+    out = np.zeros(32, dtype=np.uint8)
+    _check(lib().hs_gpu_msm_bytes(CURVES[curve], _ptr(out), _ptr(s), _ptr(p), n, flags))
+    return out.tobytes()
+
+
+def map_to_curve(curve, u, want_points=True, want_bytes=False):
+    """SSWU hash-to-curve of n 32-byte inputs on the GPU.  Returns (jacobian uint64 (n,15) or
+    None, compressed bytes or None)."""
+    ub = np.frombuffer(bytes(u), dtype=np.uint8)
+    n = ub.size // 32
+    pts = np.zeros((n, 15), dtype=np.uint64) if want_points else None
+    enc = np.zeros(n * 32, dtype=np.uint8) if want_bytes else None
+    _check(lib().hs_gpu_map_to_curve(CURVES[curve], _ptr(ub), n, _ptr(pts), _ptr(enc)))
+    return pts, (enc.tobytes() if enc is not None else None)
+
+
+def map_to_curve_device(curve, d_u, n, d_points, d_out32=None, stream=None):
+    _check(lib().hs_gpu_map_to_curve_device(CURVES[curve], d_u, n, d_points, d_out32, stream))
+
+
+def profile(on=True):
+    _check(lib().hs_gpu_profile(1 if on else 0))
+
+
+def phase_times():
+    """{phase: ms} of the last plain MSM (after profile(True))."""
+    buf = (ctypes.c_float * 16)()
+    k = lib().hs_gpu_phase_times(buf, 16)
+    return {lib().hs_gpu_phase_name(i).decode(): float(buf[i]) for i in range(k)}
+
+
+def probe_int_mul_peak():
+    """Measured 32x32->64 multiply throughput of the device (partial products per second)."""
+    v = ctypes.c_double()
+    _check(lib().hs_gpu_probe_int_mul_peak(ctypes.byref(v)))
+    return v.value
+
+
+def debug_dump(curve, which, max_items):
+    """Intermediate state of the last plain MSM: which 0/1/2 -> list of 32-byte encodings of the
+    window / segment / bucket sums; 3/4 -> uint32 array of sorted entries / bucket offsets."""
+    item = 32 if which < 3 else 4
+    buf = np.zeros(max_items * item, dtype=np.uint8)
+    cnt = ctypes.c_size_t()
+    _check(lib().hs_gpu_debug_dump(CURVES[curve], which, _ptr(buf), buf.size, ctypes.byref(cnt)))
+    if which < 3:
+        return [buf[32 * i:32 * i + 32].tobytes() for i in range(cnt.value)]
+    return buf.view(np.uint32)[:cnt.value].copy()

@@ -252,14 +252,17 @@ template <class F> HS_HD bool xyzz_to_aff(aff &r, const xyzz &p)
 }
 
 // 32-byte compressed encoding (reference helios_tobytes.cpp:42-66): canonical x little-endian,
-// bit 255 = parity of canonical y, identity = 32 zero bytes.  `a` must be canonical.
-HS_HD void encode_aff(uint32_t out[8], const aff &a, bool is_point)
+// bit 255 = parity of canonical y, identity = 32 zero bytes.
+template <class F> HS_HD void encode_aff(uint32_t out[8], const aff &a, bool is_point)
 {
+    uint32_t xw[8], yw[8];
+    fe_to_words_canon<F>(xw, a.x);
+    fe_to_words_canon<F>(yw, a.y);
 #pragma unroll
     for (int i = 0; i < 8; i++)
-        out[i] = is_point ? a.x.v[i] : 0u;
+        out[i] = is_point ? xw[i] : 0u;
     if (is_point)
-        out[7] |= (a.y.v[0] & 1u) << 31;
+        out[7] |= (yw[0] & 1u) << 31;
 }
 
 } // namespace hs

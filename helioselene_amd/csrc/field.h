@@ -1,15 +1,18 @@
 // 255-bit prime-field arithmetic for the two Helios/Selene coordinate fields, written for
-// CDNA4 VALU: eight 32-bit limbs, 32x32->64 partial products on v_mad_u64_u32 chains, and a
-// dedicated reduction per modulus (no MFMA: this is exact integer work).
+// CDNA4 VALU (no MFMA: this is exact integer work).
 //
 //   Fp: p = 2^255 - 19                (reference fp/include/x64/fp51_inline.h:46-243, fp_ops.h:42-91)
 //   Fq: q = 2^255 - gamma, gamma < 2^127 (reference fq/include/x64/fq51.h:37-86,
 //                                          fq51_inline.h:1086-1225, fq_ops.h:50-101)
 //
-// Representation: a value is any 256-bit integer v < 2^256 standing for v mod m ("redundant"
-// form).  Since 2^256 = D (mod m) with D = 38 for p and D = 2*gamma (128 bits) for q, every
-// operation folds its overflow back with D.  Only canonical() maps to [0, m); equality and
-// zero tests canonicalise first.  The reference keeps lazy radix-2^51 limbs instead; both denote
+// Representation: nine 29-bit limbs, value = sum v[i] 2^(29 i) < 2^261 + small, standing for
+// value mod m.  "Normalised" means every limb <= 2^29.  A product is formed column by column:
+// each of the 81 partial products a_i * b_j is one v_mad_u64_u32 accumulating straight into a
+// 64-bit column register (9 * 2^58 < 2^62, so a column never overflows and needs no carry
+// handling until the end).  The reduction then uses 2^261 = R (mod m): R = 1216 for p
+// (2^261 = 2^6 * 19) and R = 64 * gamma (133 bits, five limbs) for q -- the Crandall fold.
+// Only fe_to_words_canon() produces the canonical residue in [0, m) (8 x 32-bit words); equality
+// and zero tests go through it.  The reference keeps lazy radix-2^51 limbs instead; both denote
 // the same residues, so encoded outputs are identical.
 #pragma once
 #include <stdint.h>
@@ -21,79 +24,105 @@
 namespace hs
 {
 
+constexpr int NL = 9;
+constexpr uint32_t M29 = 0x1fffffffu;
+
 struct fe
 {
-    uint32_t v[8];
+    uint32_t v[NL];
 };
+
+HS_HD uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
 
 struct Fp
 {
-    // p = 2^255 - 19
-    static constexpr uint32_t M0 = 0xffffffedu, M1 = 0xffffffffu, M2 = 0xffffffffu, M3 = 0xffffffffu;
-    static constexpr uint32_t M4 = 0xffffffffu, M5 = 0xffffffffu, M6 = 0xffffffffu, M7 = 0x7fffffffu;
-    // D = 2^256 mod p (one limb), C = 2^255 mod p
-    static constexpr int DLIMBS = 1;
-    static constexpr uint32_t D0 = 38, D1 = 0, D2 = 0, D3 = 0;
-    static constexpr uint32_t C0 = 19, C1 = 0, C2 = 0, C3 = 0;
+    static constexpr int NR = 1; // limbs of R = 2^261 mod p
+    static HS_HD uint32_t mod(int i) { return i == 0 ? 0x1fffffedu : i == 8 ? 0x7fffffu : 0x1fffffffu; }
+    static HS_HD uint32_t r(int i) { return i == 0 ? 1216u : 0u; }
+    // 129 * p with every limb in [2^30, 2^31): subtraction bias
+    static HS_HD uint32_t bias(int i)
+    {
+        return i == 0 ? 0x5ffff66du : i == 8 ? 0x407ffffdu : 0x5ffffffdu;
+    }
+    // modulus and 2^255 mod m as 32-bit words (canonicalisation)
+    static HS_HD uint32_t mw(int i) { return i == 0 ? 0xffffffedu : i == 7 ? 0x7fffffffu : 0xffffffffu; }
+    static HS_HD uint32_t cw(int i) { return i == 0 ? 19u : 0u; }
+    static constexpr int NCW = 1;
 };
 
 struct Fq
 {
+    static constexpr int NR = 5;
+    static HS_HD uint32_t mod(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x1927c79fu;
+        case 1: return 0x15b69393u;
+        case 2: return 0x195ad61bu;
+        case 3: return 0x1ef0596eu;
+        case 4: return 0x1ffffbf7u;
+        case 8: return 0x7fffffu;
+        default: return 0x1fffffffu;
+        }
+    }
+    // R = 2^261 mod q = 64 * gamma
+    static HS_HD uint32_t r(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x160e1840u;
+        case 1: return 0x125b1b0du;
+        case 2: return 0x094a7914u;
+        case 3: return 0x03e9a44du;
+        case 4: return 0x00010202u;
+        default: return 0u;
+        }
+    }
+    static HS_HD uint32_t bias(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x4d0b971fu;
+        case 1: return 0x51005d76u;
+        case 2: return 0x46c5e3f0u;
+        case 3: return 0x571d10d2u;
+        case 4: return 0x5ffdf7f1u;
+        case 8: return 0x407ffffdu;
+        default: return 0x5ffffffdu;
+        }
+    }
     // q = 2^255 - gamma, gamma = 0x408087d3489a94a791492d8d86d83861 (reference fq51.h:39-62)
-    static constexpr uint32_t M0 = 0x7927c79fu, M1 = 0x6eb6d272u, M2 = 0xb7656b58u, M3 = 0xbf7f782cu;
-    static constexpr uint32_t M4 = 0xffffffffu, M5 = 0xffffffffu, M6 = 0xffffffffu, M7 = 0x7fffffffu;
-    static constexpr int DLIMBS = 4;
-    // D = 2*gamma = 0x81010fa69135294f22925b1b0db070c2
-    static constexpr uint32_t D0 = 0x0db070c2u, D1 = 0x22925b1bu, D2 = 0x9135294fu, D3 = 0x81010fa6u;
-    // C = gamma
-    static constexpr uint32_t C0 = 0x86d83861u, C1 = 0x91492d8du, C2 = 0x489a94a7u, C3 = 0x408087d3u;
+    static HS_HD uint32_t mw(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x7927c79fu;
+        case 1: return 0x6eb6d272u;
+        case 2: return 0xb7656b58u;
+        case 3: return 0xbf7f782cu;
+        case 7: return 0x7fffffffu;
+        default: return 0xffffffffu;
+        }
+    }
+    static HS_HD uint32_t cw(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x86d83861u;
+        case 1: return 0x91492d8du;
+        case 2: return 0x489a94a7u;
+        case 3: return 0x408087d3u;
+        default: return 0u;
+        }
+    }
+    static constexpr int NCW = 4;
 };
-
-template <class F> HS_HD uint32_t mod_limb(int i)
-{
-    switch (i)
-    {
-    case 0: return F::M0;
-    case 1: return F::M1;
-    case 2: return F::M2;
-    case 3: return F::M3;
-    case 4: return F::M4;
-    case 5: return F::M5;
-    case 6: return F::M6;
-    default: return F::M7;
-    }
-}
-
-template <class F> HS_HD uint32_t d_limb(int i)
-{
-    switch (i)
-    {
-    case 0: return F::D0;
-    case 1: return F::D1;
-    case 2: return F::D2;
-    case 3: return F::D3;
-    default: return 0;
-    }
-}
-
-template <class F> HS_HD uint32_t c_limb(int i)
-{
-    switch (i)
-    {
-    case 0: return F::C0;
-    case 1: return F::C1;
-    case 2: return F::C2;
-    case 3: return F::C3;
-    default: return 0;
-    }
-}
-
-HS_HD uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
 
 HS_HD void fe_zero(fe &r)
 {
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < NL; i++)
         r.v[i] = 0;
 }
 
@@ -101,76 +130,76 @@ HS_HD void fe_one(fe &r)
 {
     r.v[0] = 1;
 #pragma unroll
-    for (int i = 1; i < 8; i++)
+    for (int i = 1; i < NL; i++)
         r.v[i] = 0;
 }
 
-// r += k*D (mod 2^256) for a small multiplier k (k*D*... < 2^32 per limb product); returns the
-// carry out of bit 256.
-template <class F> HS_HD uint32_t add_kd(fe &r, uint32_t k)
+HS_HD void fe_small(fe &r, uint32_t k)
 {
-    uint64_t c = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-    {
-        if (i < F::DLIMBS)
-            c = mad64(k, d_limb<F>(i), (uint64_t)r.v[i] + (c >> 32));
-        else
-            c = (uint64_t)r.v[i] + (c >> 32);
-        r.v[i] = (uint32_t)c;
-    }
-    return (uint32_t)(c >> 32);
+    fe_zero(r);
+    r.v[0] = k;
 }
 
-// Final fold shared by every operation: value = r + carry*2^256 with carry small; result < 2^256.
-template <class F> HS_HD void fold_carry(fe &r, uint32_t carry)
+// r (limbs < 2^29, plus the value c * 2^261 carried out of the top) -> normalised.
+// Adds c * R, c small (< 2^14).
+template <class F> HS_HD void fold_top(fe &r, uint32_t c)
 {
-    uint32_t c2 = add_kd<F>(r, carry);
-    // A second carry leaves r < k*D, so adding D once more cannot overflow.
-    add_kd<F>(r, c2);
+    if (F::NR == 1)
+    {
+        uint32_t t = r.v[0] + c * F::r(0);
+        r.v[0] = t & M29;
+        r.v[1] += t >> 29; // r.v[1] <= 2^29 afterwards
+    }
+    else
+    {
+        // two passes: the second carry (0 or 1) cannot ripple past limb 5
+        uint64_t t = 0;
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+        {
+            t = (i < F::NR ? mad64(c, F::r(i), r.v[i]) : (uint64_t)r.v[i]) + (t >> 29);
+            r.v[i] = (uint32_t)t & M29;
+        }
+        uint32_t c2 = (uint32_t)(t >> 29);
+        uint32_t u = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+        {
+            u = (i < F::NR ? c2 * F::r(i) + r.v[i] : r.v[i]) + (u >> 29);
+            r.v[i] = u & M29;
+        }
+        r.v[6] += u >> 29;
+    }
+}
+
+// limbs < 2^31.7 -> normalised
+template <class F> HS_HD void fe_norm(fe &r)
+{
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        uint32_t t = r.v[i] + c;
+        r.v[i] = t & M29;
+        c = t >> 29;
+    }
+    fold_top<F>(r, c);
 }
 
 template <class F> HS_HD void fe_add(fe &r, const fe &a, const fe &b)
 {
-    uint64_t c = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-    {
-        c = (uint64_t)a.v[i] + b.v[i] + (c >> 32);
-        r.v[i] = (uint32_t)c;
-    }
-    fold_carry<F>(r, (uint32_t)(c >> 32));
-}
-
-// r -= k*D (mod 2^256); returns the borrow out.
-template <class F> HS_HD uint32_t sub_kd(fe &r, uint32_t k)
-{
-    int64_t b = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-    {
-        uint64_t sub = (i < F::DLIMBS) ? (uint64_t)k * d_limb<F>(i) : 0;
-        int64_t t = (int64_t)r.v[i] - (int64_t)(sub & 0xffffffffu) + b;
-        r.v[i] = (uint32_t)t;
-        b = (t >> 32) - (int64_t)(sub >> 32);
-    }
-    return (uint32_t)(-b);
+    for (int i = 0; i < NL; i++)
+        r.v[i] = a.v[i] + b.v[i];
+    fe_norm<F>(r);
 }
 
 template <class F> HS_HD void fe_sub(fe &r, const fe &a, const fe &b)
 {
-    int64_t br = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-    {
-        int64_t t = (int64_t)a.v[i] - b.v[i] + br;
-        r.v[i] = (uint32_t)t;
-        br = t >> 32;
-    }
-    // a wrapped borrow added 2^256 = D (mod m): take D back off, at most twice
-    uint32_t b1 = (uint32_t)(-br);
-    uint32_t b2 = sub_kd<F>(r, b1);
-    sub_kd<F>(r, b2);
+    for (int i = 0; i < NL; i++)
+        r.v[i] = a.v[i] + F::bias(i) - b.v[i];
+    fe_norm<F>(r);
 }
 
 template <class F> HS_HD void fe_neg(fe &r, const fe &a)
@@ -180,195 +209,211 @@ template <class F> HS_HD void fe_neg(fe &r, const fe &a)
     fe_sub<F>(r, z, a);
 }
 
-// r = a * 2 (a + a)
 template <class F> HS_HD void fe_dbl(fe &r, const fe &a) { fe_add<F>(r, a, a); }
 
-// Reduce a 512-bit product t[16] to < 2^256 using 2^256 = D (mod m).
-template <class F> HS_HD void reduce512(fe &r, const uint32_t t[16])
+// 17 product columns (each < 2^62) -> normalised residue
+template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
 {
-    if (F::DLIMBS == 1)
-    {
-        // r = t_lo + D * t_hi  (D = 38)
-        uint64_t c = 0;
+    // full carry propagation: 29-bit limbs l[0..16] and l[17] = the rest (< 2^30)
+    uint32_t l[2 * NL];
+    uint64_t t = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++)
+    for (int k = 0; k < 2 * NL - 1; k++)
+    {
+        t = c[k] + (t >> 29);
+        l[k] = (uint32_t)t & M29;
+    }
+    l[2 * NL - 1] = (uint32_t)(t >> 29);
+    if (F::NR == 1)
+    {
+        // w_k = l_k + R * l_{9+k}
+        uint64_t u = 0;
+#pragma unroll
+        for (int k = 0; k < NL; k++)
         {
-            c = mad64(t[8 + j], F::D0, (uint64_t)t[j] + (c >> 32));
-            r.v[j] = (uint32_t)c;
+            u = mad64(l[NL + k], F::r(0), (uint64_t)l[k] + (u >> 29));
+            r.v[k] = (uint32_t)u & M29;
         }
-        fold_carry<F>(r, (uint32_t)(c >> 32));
+        fold_top<F>(r, (uint32_t)(u >> 29));
     }
     else
     {
-        // first fold: w = t_lo + t_hi * D, 12 limbs (t_hi: 8 limbs, D: 4 limbs)
-        uint32_t w[12];
+        // first fold: 9 high limbs x 5 limbs of R into columns 0..12
+        uint64_t w[NL + F::NR - 1];
 #pragma unroll
-        for (int j = 0; j < 12; j++)
-            w[j] = j < 8 ? t[j] : 0;
+        for (int k = 0; k < NL + F::NR - 1; k++)
+            w[k] = k < NL ? l[k] : 0;
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NL; i++)
+#pragma unroll
+            for (int j = 0; j < F::NR; j++)
+                w[i + j] = mad64(l[NL + i], F::r(j), w[i + j]);
+        uint32_t m[NL + F::NR];
+        uint64_t u = 0;
+#pragma unroll
+        for (int k = 0; k < NL + F::NR - 1; k++)
         {
-            uint64_t c = 0;
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-            {
-                c = mad64(t[8 + j], d_limb<F>(i), (uint64_t)w[i + j] + (c >> 32));
-                w[i + j] = (uint32_t)c;
-            }
-            // propagate the row carry into the remaining limbs
-            uint64_t cc = c >> 32;
-#pragma unroll
-            for (int k = i + 8; k < 12; k++)
-            {
-                cc += w[k];
-                w[k] = (uint32_t)cc;
-                cc >>= 32;
-            }
+            u = w[k] + (u >> 29);
+            m[k] = (uint32_t)u & M29;
         }
-        // second fold: r = w_lo + w_hi * D, w_hi = w[8..11] (4 limbs)
-        uint32_t x[9];
+        m[NL + F::NR - 1] = (uint32_t)(u >> 29);
+        // second fold: limbs 9..13 (about 133 bits) x R into columns 0..8
+        uint64_t x[NL];
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-            x[j] = w[j];
-        x[8] = 0;
+        for (int k = 0; k < NL; k++)
+            x[k] = m[k];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < F::NR; i++)
+#pragma unroll
+            for (int j = 0; j < F::NR; j++)
+                if (i + j < NL)
+                    x[i + j] = mad64(m[NL + i], F::r(j), x[i + j]);
+        u = 0;
+#pragma unroll
+        for (int k = 0; k < NL; k++)
         {
-            uint64_t c = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-            {
-                c = mad64(w[8 + j], d_limb<F>(i), (uint64_t)x[i + j] + (c >> 32));
-                x[i + j] = (uint32_t)c;
-            }
-            uint64_t cc = c >> 32;
-#pragma unroll
-            for (int k = i + 4; k < 9; k++)
-            {
-                cc += x[k];
-                x[k] = (uint32_t)cc;
-                cc >>= 32;
-            }
+            u = x[k] + (u >> 29);
+            r.v[k] = (uint32_t)u & M29;
         }
-#pragma unroll
-        for (int j = 0; j < 8; j++)
-            r.v[j] = x[j];
-        fold_carry<F>(r, x[8]);
+        fold_top<F>(r, (uint32_t)(u >> 29));
     }
 }
 
 template <class F> HS_HD void fe_mul(fe &r, const fe &a, const fe &b)
 {
-    uint32_t t[16];
-    {
-        uint64_t c = 0;
+    uint64_t c[2 * NL - 1];
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-        {
-            c = mad64(a.v[0], b.v[j], c >> 32);
-            t[j] = (uint32_t)c;
-        }
-        t[8] = (uint32_t)(c >> 32);
-    }
+    for (int k = 0; k < 2 * NL - 1; k++)
+        c[k] = 0;
 #pragma unroll
-    for (int i = 1; i < 8; i++)
-    {
-        uint64_t c = 0;
+    for (int i = 0; i < NL; i++)
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-        {
-            c = mad64(a.v[i], b.v[j], (uint64_t)t[i + j] + (c >> 32));
-            t[i + j] = (uint32_t)c;
-        }
-        t[i + 8] = (uint32_t)(c >> 32);
-    }
-    reduce512<F>(r, t);
+        for (int j = 0; j < NL; j++)
+            c[i + j] = mad64(a.v[i], b.v[j], c[i + j]);
+    reduce_cols<F>(r, c);
 }
 
 template <class F> HS_HD void fe_sq(fe &r, const fe &a)
 {
-    uint32_t t[16];
+    uint64_t c[2 * NL - 1];
+    uint32_t d[NL];
 #pragma unroll
-    for (int i = 0; i < 16; i++)
-        t[i] = 0;
-    // off-diagonal products a_i a_j, i < j
+    for (int k = 0; k < 2 * NL - 1; k++)
+        c[k] = 0;
 #pragma unroll
-    for (int i = 0; i < 7; i++)
+    for (int i = 0; i < NL; i++)
+        d[i] = a.v[i] << 1;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
     {
-        uint64_t c = 0;
+        c[2 * i] = mad64(a.v[i], a.v[i], c[2 * i]);
 #pragma unroll
-        for (int j = i + 1; j < 8; j++)
-        {
-            c = mad64(a.v[i], a.v[j], (uint64_t)t[i + j] + (c >> 32));
-            t[i + j] = (uint32_t)c;
-        }
-        t[i + 8] = (uint32_t)(c >> 32);
+        for (int j = i + 1; j < NL; j++)
+            c[i + j] = mad64(a.v[i], d[j], c[i + j]);
     }
-    // double
-    t[15] = t[14] >> 31;
-#pragma unroll
-    for (int i = 14; i > 0; i--)
-        t[i] = (t[i] << 1) | (t[i - 1] >> 31);
-    t[0] <<= 1;
-    // add the squares a_i^2 at limb 2i
-    uint64_t c = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-    {
-        uint64_t s = (uint64_t)a.v[i] * a.v[i];
-        c = (uint64_t)t[2 * i] + (uint32_t)s + (c >> 32);
-        t[2 * i] = (uint32_t)c;
-        c = (uint64_t)t[2 * i + 1] + (uint32_t)(s >> 32) + (c >> 32);
-        t[2 * i + 1] = (uint32_t)c;
-    }
-    reduce512<F>(r, t);
+    reduce_cols<F>(r, c);
 }
 
-// small-constant multiply r = a * k (k < 2^32)
+// r = a * k for a small constant k (< 2^29)
 template <class F> HS_HD void fe_mul_small(fe &r, const fe &a, uint32_t k)
 {
-    uint64_t c = 0;
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        t = mad64(a.v[i], k, t >> 29);
+        r.v[i] = (uint32_t)t & M29;
+    }
+    fold_top<F>(r, (uint32_t)(t >> 29));
+}
+
+// ---------------------------------------------------------------- conversions
+// 8 little-endian words (any value < 2^256) -> limbs (normalised, not reduced)
+HS_HD void fe_from_words(fe &r, const uint32_t w[8])
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        const int bit = 29 * i, wi = bit >> 5, sh = bit & 31;
+        uint64_t lo = wi < 8 ? w[wi] : 0u;
+        uint64_t hi = wi + 1 < 8 ? w[wi + 1] : 0u;
+        r.v[i] = (uint32_t)(((lo | (hi << 32)) >> sh) & M29);
+    }
+}
+
+// canonical residue in [0, m) as 8 little-endian words
+template <class F> HS_HD void fe_to_words_canon(uint32_t out[8], const fe &a)
+{
+    // value = sum v[i] 2^(29 i) with v[i] <= 2^29 + 1: rebuild 9 words exactly
+    uint32_t w[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        w[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        const int bit = 29 * i, wi = bit >> 5, sh = bit & 31;
+        uint64_t x = (uint64_t)a.v[i] << sh; // < 2^62
+        uint64_t t = (uint64_t)w[wi] + (uint32_t)x;
+        w[wi] = (uint32_t)t;
+        t = (t >> 32) + (x >> 32);
+#pragma unroll
+        for (int k = wi + 1; k < 9; k++)
+        {
+            t += w[k];
+            w[k] = (uint32_t)t;
+            t >>= 32;
+        }
+    }
+    // fold bits >= 255: value = L + H * 2^255 = L + H * (2^255 mod m), H < 2^8
+    uint32_t H = (w[7] >> 31) | (w[8] << 1);
+    w[7] &= 0x7fffffffu;
+    uint64_t t = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++)
     {
-        c = mad64(a.v[i], k, c >> 32);
-        r.v[i] = (uint32_t)c;
+        t = (i < F::NCW ? mad64(H, F::cw(i), w[i]) : (uint64_t)w[i]) + (t >> 32);
+        w[i] = (uint32_t)t;
     }
-    fold_carry<F>(r, (uint32_t)(c >> 32));
-}
-
-// Canonical residue in [0, m): v < 2^256 < 2m + D, so at most two subtractions of m.
-template <class F> HS_HD void fe_canon(fe &r, const fe &a)
-{
-    r = a;
+    // now value < 2^255 + 2^8 * 2^127 < 2m: subtract m once if needed (twice for safety)
 #pragma unroll
     for (int pass = 0; pass < 2; pass++)
     {
-        // t = r - m; keep it when there is no borrow
-        uint32_t t[8];
+        uint32_t d[8];
         int64_t br = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++)
         {
-            int64_t x = (int64_t)r.v[i] - mod_limb<F>(i) + br;
-            t[i] = (uint32_t)x;
+            int64_t x = (int64_t)w[i] - F::mw(i) + br;
+            d[i] = (uint32_t)x;
             br = x >> 32;
         }
-        uint32_t keep = br == 0 ? 0xffffffffu : 0u;
+        const uint32_t keep = br == 0 ? 0xffffffffu : 0u;
 #pragma unroll
         for (int i = 0; i < 8; i++)
-            r.v[i] = (t[i] & keep) | (r.v[i] & ~keep);
+            w[i] = (d[i] & keep) | (w[i] & ~keep);
     }
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        out[i] = w[i];
+}
+
+// canonical (fully reduced) limbs
+template <class F> HS_HD void fe_canon(fe &r, const fe &a)
+{
+    uint32_t w[8];
+    fe_to_words_canon<F>(w, a);
+    fe_from_words(r, w);
 }
 
 template <class F> HS_HD bool fe_is_zero(const fe &a)
 {
-    fe c;
-    fe_canon<F>(c, a);
+    uint32_t w[8];
+    fe_to_words_canon<F>(w, a);
     uint32_t o = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++)
-        o |= c.v[i];
+        o |= w[i];
     return o == 0;
 }
 
@@ -377,6 +422,13 @@ template <class F> HS_HD bool fe_eq(const fe &a, const fe &b)
     fe d;
     fe_sub<F>(d, a, b);
     return fe_is_zero<F>(d);
+}
+
+template <class F> HS_HD bool fe_is_odd(const fe &a)
+{
+    uint32_t w[8];
+    fe_to_words_canon<F>(w, a);
+    return w[0] & 1u;
 }
 
 // Exponentiation by a fixed 256-bit exponent given as 8 little-endian 32-bit words, 4-bit
@@ -402,9 +454,7 @@ template <class F> HS_HD void fe_pow(fe &r, const fe &a, const uint32_t e[8])
             fe_sq<F>(acc, acc);
         }
         uint32_t nib = (e[w >> 3] >> ((w & 7) * 4)) & 15u;
-        fe t;
-        // select without a data-dependent index into registers
-        t = tab[0];
+        fe t = tab[0];
 #pragma unroll
         for (int k = 1; k < 16; k++)
             if (nib == (uint32_t)k)
@@ -414,41 +464,19 @@ template <class F> HS_HD void fe_pow(fe &r, const fe &a, const uint32_t e[8])
     r = acc;
 }
 
-// m - 2 as an exponent
-template <class F> HS_HD void exp_m_minus_2(uint32_t e[8])
-{
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-        e[i] = mod_limb<F>(i);
-    e[0] -= 2; // both moduli have a low limb >= 2
-}
-
-// a^-1 (Fermat, a^(m-2)); maps 0 to 0 like the reference's fp_invert / fq_invert.
+// a^-1 = a^(m-2) (Fermat); maps 0 to 0 like the reference's fp_invert / fq_invert.
 template <class F> HS_HD void fe_inv(fe &r, const fe &a)
 {
     uint32_t e[8];
-    exp_m_minus_2<F>(e);
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        e[i] = F::mw(i);
+    e[0] -= 2; // both moduli have a low word >= 2
     fe_pow<F>(r, a, e);
 }
 
-template <class F> HS_HD bool fe_is_odd(const fe &a)
-{
-    fe c;
-    fe_canon<F>(c, a);
-    return c.v[0] & 1;
-}
-
-// --------------------------------------------------------------- conversions
-// 32-byte little-endian -> fe (value < 2^256, not reduced)
-HS_HD void fe_from_words(fe &r, const uint32_t w[8])
-{
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-        r.v[i] = w[i];
-}
-
 // radix-2^51 limbs (the reference's fp_fe / fq_fe, uint64_t[5], limbs may exceed 51 bits) ->
-// reduced fe.  value = sum l_i 2^(51 i) < 2^(204 + 64).
+// normalised fe.  value = sum l_i 2^(51 i) < 2^(204 + 64).
 template <class F> HS_HD void fe_from_r51(fe &r, const uint64_t l[5])
 {
     uint32_t w[10];
@@ -459,9 +487,8 @@ template <class F> HS_HD void fe_from_r51(fe &r, const uint64_t l[5])
     for (int i = 0; i < 5; i++)
     {
         const int bit = 51 * i, lw = bit >> 5, sh = bit & 31;
-        // add l_i << sh into words lw, lw+1, lw+2 (and carry on)
-        uint64_t lo = l[i] << sh;                        // low 64 bits of the shifted limb
-        uint64_t hi = sh ? (l[i] >> (64 - sh)) : 0;      // bits above 64
+        uint64_t lo = l[i] << sh;
+        uint64_t hi = sh ? (l[i] >> (64 - sh)) : 0;
         uint64_t c = (uint64_t)w[lw] + (uint32_t)lo;
         w[lw] = (uint32_t)c;
         c = (uint64_t)w[lw + 1] + (uint32_t)(lo >> 32) + (c >> 32);
@@ -476,25 +503,23 @@ template <class F> HS_HD void fe_from_r51(fe &r, const uint64_t l[5])
             c >>= 32;
         }
     }
-    // value = w[0..7] + 2^256 * (w[8] + 2^32 w[9]); 2^256 = D, and w[8..9] < 2^13
-    fe lo;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-        lo.v[i] = w[i];
-    // w[9] is zero: the input is below 2^268
-    fold_carry<F>(lo, w[8]);
-    r = lo;
+    // value = w[0..7] + 2^256 * w[8] (w[8] < 2^12, w[9] = 0); 2^256 = 2^-5 * 2^261 is not a limb
+    // boundary, so fold through 2^261: 2^256 * w8 = 2^261 * (w8 >> 5) + 2^256 * (w8 & 31)
+    fe_from_words(r, w);
+    // w8 & 31 goes to bits 256..260 of limb 8 (bit 24..28 of limb 8)
+    r.v[8] |= (w[8] & 31u) << 24;
+    fold_top<F>(r, w[8] >> 5);
 }
 
 // fe -> canonical radix-2^51 limbs
 template <class F> HS_HD void fe_to_r51(uint64_t l[5], const fe &a)
 {
-    fe c;
-    fe_canon<F>(c, a);
+    uint32_t c[8];
+    fe_to_words_canon<F>(c, a);
     uint64_t w[4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
-        w[i] = (uint64_t)c.v[2 * i] | ((uint64_t)c.v[2 * i + 1] << 32);
+        w[i] = (uint64_t)c[2 * i] | ((uint64_t)c[2 * i + 1] << 32);
     const uint64_t M = (1ull << 51) - 1;
     l[0] = w[0] & M;
     l[1] = ((w[0] >> 51) | (w[1] << 13)) & M;

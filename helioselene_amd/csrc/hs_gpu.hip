@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -113,8 +114,19 @@ struct Ctx
     hipStream_t stream = nullptr;
     std::mutex mu;
     DevBuf scalars, points, aff, isid, digits, counts, offsets, block_sums, sorted, buckets, seg, win, big_list,
-        big_count, out32, outjac;
+        big_count, out32, outjac, status, nbad, probe;
     bool ready = false;
+    // phase timing (hs_gpu_profile): events recorded between the phases of the last plain MSM
+    bool profile = false;
+    static constexpr int NEV = 8;
+    hipEvent_t ev[NEV] = {};
+    int nev = 0;
+
+    void mark(hipStream_t st)
+    {
+        if (profile && nev < NEV)
+            hipEventRecord(ev[nev++], st);
+    }
 
     int init()
     {
@@ -128,6 +140,8 @@ struct Ctx
             return fail(HS_GPU_EDEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
         device = dev;
         HS_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (int i = 0; i < NEV; i++)
+            HS_CHECK(hipEventCreate(&ev[i]));
         ready = true;
         return 0;
     }
@@ -174,7 +188,7 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         (r = cx.seg.reserve((size_t)P.segs * sizeof(hs::xyzz))) || (r = cx.win.reserve((size_t)P.W * sizeof(hs::xyzz))) ||
         (r = cx.big_list.reserve((size_t)P.nb * 4)) || (r = cx.big_count.reserve(64)))
         return r;
-    hs::aff *aff = cx.aff.as<hs::aff>();
+    uint32_t *aff = cx.aff.as<uint32_t>();
     uint8_t *isid = cx.isid.as<uint8_t>();
     uint32_t *digits = cx.digits.as<uint32_t>(), *counts = cx.counts.as<uint32_t>(),
              *offsets = cx.offsets.as<uint32_t>(), *sorted = cx.sorted.as<uint32_t>();
@@ -182,24 +196,59 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     uint32_t *big_list = cx.big_list.as<uint32_t>(), *big_count = cx.big_count.as<uint32_t>();
 
     const dim3 blk(256), grd((nn + 255) / 256);
+    cx.nev = 0;
+    cx.mark(st);
     hipLaunchKernelGGL(hs::k_ingest<C>, grd, blk, 0, st, d_points, nn, aff, isid);
+    cx.mark(st);
     HS_CHECK(hipMemsetAsync(counts, 0, (size_t)(P.nb + 1) * 4, st));
     HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
     hipLaunchKernelGGL(hs::k_recode<C>, grd, blk, 0, st, d_scalars, isid, nn, P.c, P.W, compat ? 1 : 0, digits, counts);
+    cx.mark(st);
     if ((r = scan_exclusive(cx, counts, offsets, P.nb + 1, st)))
         return r;
     // cursor = offsets (counts reused as the cursor array)
     HS_CHECK(hipMemcpyAsync(counts, offsets, (size_t)P.nb * 4, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(hs::k_scatter, grd, blk, 0, st, digits, nn, P.W, P.B, counts, sorted);
+    cx.mark(st);
     const uint32_t lmax = 96;
     hipLaunchKernelGGL(hs::k_bucket_small<C>, dim3((P.nb + 255) / 256), blk, 0, st, offsets, P.nb, sorted, aff, lmax,
                        buckets, big_list, big_count);
     hipLaunchKernelGGL(hs::k_bucket_big<C>, dim3(512), dim3(hs::BIG_THREADS), 0, st, offsets, sorted, aff, big_list,
                        big_count, buckets);
+    cx.mark(st);
     hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((P.segs + 255) / 256), blk, 0, st, buckets, P.B, P.G, P.segs, seg);
     hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(P.W), dim3(hs::BIG_THREADS), 0, st, seg, P.segs / P.W, win);
+    cx.mark(st);
     hipLaunchKernelGGL(hs::k_combine<C>, dim3(1), dim3(64), 0, st, win, P.W, P.c, d_out32, d_outjac);
+    cx.mark(st);
     HS_CHECK(hipGetLastError());
+    return 0;
+}
+
+const char *const PHASE_NAMES[] = {"ingest", "recode", "sort", "bucket", "reduce", "combine"};
+
+template <class C>
+int map_to_curve_device(Ctx &cx, const uint32_t *d_u, size_t n, uint64_t *d_jac, uint32_t *d_out32, hipStream_t st)
+{
+    (void)cx;
+    const uint32_t nn = (uint32_t)n;
+    hipLaunchKernelGGL(hs::k_map_to_curve<C>, dim3((nn + 255) / 256), dim3(256), 0, st, d_u, nn, d_jac, d_out32);
+    HS_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <class C>
+int decompress_device(Ctx &cx, const uint32_t *d_in32, size_t n, uint64_t *d_jac, hipStream_t st, uint32_t *h_nbad)
+{
+    int r;
+    if ((r = cx.nbad.reserve(64)))
+        return r;
+    const uint32_t nn = (uint32_t)n;
+    HS_CHECK(hipMemsetAsync(cx.nbad.p, 0, 4, st));
+    hipLaunchKernelGGL(hs::k_decompress<C>, dim3((nn + 255) / 256), dim3(256), 0, st, d_in32, nn, d_jac,
+                       (uint8_t *)nullptr, cx.nbad.as<uint32_t>());
+    HS_CHECK(hipGetLastError());
+    HS_CHECK(hipMemcpyAsync(h_nbad, cx.nbad.p, 4, hipMemcpyDeviceToHost, st));
     return 0;
 }
 
@@ -328,6 +377,192 @@ int hs_gpu_init(void)
 }
 
 const char *hs_gpu_last_error(void) { return g_err.c_str(); }
+
+int hs_gpu_profile(int on)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()))
+        return r;
+    cx.profile = on != 0;
+    cx.nev = 0;
+    return 0;
+}
+
+int hs_gpu_phase_times(float *ms, int cap)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    if (!cx.ready || cx.nev < 2)
+        return 0;
+    HS_CHECK(hipEventSynchronize(cx.ev[cx.nev - 1]));
+    int k = 0;
+    for (int i = 0; i + 1 < cx.nev && k < cap; i++, k++)
+        HS_CHECK(hipEventElapsedTime(&ms[k], cx.ev[i], cx.ev[i + 1]));
+    return k;
+}
+
+const char *hs_gpu_phase_name(int i)
+{
+    return (i >= 0 && i < (int)(sizeof PHASE_NAMES / sizeof PHASE_NAMES[0])) ? PHASE_NAMES[i] : "";
+}
+
+int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t *count)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    if (!cx.ready)
+        return fail(HS_GPU_EINVAL, "no MSM has run");
+    *count = 0;
+    if (which == 3 || which == 4)
+    {
+        // 3: sorted entries, 4: bucket offsets (raw uint32)
+        DevBuf &b = which == 3 ? cx.sorted : cx.offsets;
+        size_t nb = std::min(cap_bytes, b.cap);
+        HS_CHECK(hipMemcpy(out, b.p, nb, hipMemcpyDeviceToHost));
+        *count = nb / 4;
+        return 0;
+    }
+    DevBuf &src = which == 0 ? cx.win : which == 1 ? cx.seg : cx.buckets;
+    size_t n = std::min(cap_bytes / 32, src.cap / sizeof(hs::xyzz));
+    int r;
+    if ((r = cx.probe.reserve(n * 32 + 64)))
+        return r;
+    const uint32_t nn = (uint32_t)n;
+    if (curve == HS_GPU_HELIOS)
+        hipLaunchKernelGGL(hs::k_encode_xyzz<hs::HeliosC>, dim3((nn + 63) / 64), dim3(64), 0, cx.stream,
+                           src.as<hs::xyzz>(), nn, cx.probe.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(hs::k_encode_xyzz<hs::SeleneC>, dim3((nn + 63) / 64), dim3(64), 0, cx.stream,
+                           src.as<hs::xyzz>(), nn, cx.probe.as<uint32_t>());
+    HS_CHECK(hipGetLastError());
+    HS_CHECK(hipStreamSynchronize(cx.stream));
+    HS_CHECK(hipMemcpy(out, cx.probe.p, n * 32, hipMemcpyDeviceToHost));
+    *count = n;
+    return 0;
+}
+
+int hs_gpu_probe_int_mul_peak(double *products_per_second)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()))
+        return r;
+    hipDeviceProp_t prop;
+    HS_CHECK(hipGetDeviceProperties(&prop, cx.device));
+    const int blocks = prop.multiProcessorCount * 16, threads = 256, iters = 4096;
+    if ((r = cx.probe.reserve((size_t)blocks * threads * 4)))
+        return r;
+    hipEvent_t e0, e1;
+    HS_CHECK(hipEventCreate(&e0));
+    HS_CHECK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(hs::k_probe_mad64, dim3(blocks), dim3(threads), 0, cx.stream, cx.probe.as<uint32_t>(), 64, 7u);
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; rep++)
+    {
+        HS_CHECK(hipEventRecord(e0, cx.stream));
+        hipLaunchKernelGGL(hs::k_probe_mad64, dim3(blocks), dim3(threads), 0, cx.stream, cx.probe.as<uint32_t>(),
+                           iters, 7u + rep);
+        HS_CHECK(hipEventRecord(e1, cx.stream));
+        HS_CHECK(hipEventSynchronize(e1));
+        float ms;
+        HS_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best)
+            best = ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *products_per_second = (double)blocks * threads * iters * 8 / (best * 1e-3);
+    return 0;
+}
+
+int hs_gpu_map_to_curve(int curve, const unsigned char *u, size_t n, hs_gpu_jacobian *out_points, unsigned char *out32)
+{
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        return fail(HS_GPU_EINVAL, "unknown curve id");
+    if (n == 0)
+        return 0;
+    if (!u || (!out_points && !out32))
+        return fail(HS_GPU_EINVAL, "null argument");
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()) || (r = cx.scalars.reserve(n * 32)) || (r = cx.points.reserve(n * sizeof(hs_gpu_jacobian))) ||
+        (r = cx.digits.reserve(n * 32)))
+        return r;
+    hipStream_t st = cx.stream;
+    HS_CHECK(hipMemcpyAsync(cx.scalars.p, u, n * 32, hipMemcpyHostToDevice, st));
+    uint64_t *dj = out_points ? cx.points.as<uint64_t>() : nullptr;
+    uint32_t *d32 = out32 ? cx.digits.as<uint32_t>() : nullptr;
+    r = curve == HS_GPU_HELIOS ? map_to_curve_device<hs::HeliosC>(cx, cx.scalars.as<uint32_t>(), n, dj, d32, st)
+                               : map_to_curve_device<hs::SeleneC>(cx, cx.scalars.as<uint32_t>(), n, dj, d32, st);
+    if (r)
+        return r;
+    if (out_points)
+        HS_CHECK(hipMemcpyAsync(out_points, dj, n * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToHost, st));
+    if (out32)
+        HS_CHECK(hipMemcpyAsync(out32, d32, n * 32, hipMemcpyDeviceToHost, st));
+    HS_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int hs_gpu_map_to_curve_device(int curve, const void *d_u, size_t n, void *d_points, void *d_out32, void *stream)
+{
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        return fail(HS_GPU_EINVAL, "unknown curve id");
+    if (n == 0)
+        return 0;
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()))
+        return r;
+    hipStream_t st = stream ? (hipStream_t)stream : cx.stream;
+    return curve == HS_GPU_HELIOS
+               ? map_to_curve_device<hs::HeliosC>(cx, (const uint32_t *)d_u, n, (uint64_t *)d_points, (uint32_t *)d_out32, st)
+               : map_to_curve_device<hs::SeleneC>(cx, (const uint32_t *)d_u, n, (uint64_t *)d_points, (uint32_t *)d_out32, st);
+}
+
+int hs_gpu_msm_bytes(int curve, unsigned char out32[32], const unsigned char *scalars, const unsigned char *points32,
+                     size_t n, unsigned flags)
+{
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        return fail(HS_GPU_EINVAL, "unknown curve id");
+    if (!out32)
+        return fail(HS_GPU_EINVAL, "null output");
+    if (n == 0)
+    {
+        std::memset(out32, 0, 32);
+        return 0;
+    }
+    if (!scalars || !points32)
+        return fail(HS_GPU_EINVAL, "null input with n > 0");
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()) || (r = cx.scalars.reserve(n * 32)) || (r = cx.points.reserve(n * sizeof(hs_gpu_jacobian))) ||
+        (r = cx.status.reserve(n * 32)) || (r = cx.out32.reserve(64)))
+        return r;
+    hipStream_t st = cx.stream;
+    HS_CHECK(hipMemcpyAsync(cx.scalars.p, scalars, n * 32, hipMemcpyHostToDevice, st));
+    HS_CHECK(hipMemcpyAsync(cx.status.p, points32, n * 32, hipMemcpyHostToDevice, st));
+    uint32_t nbad = 0;
+    r = curve == HS_GPU_HELIOS
+            ? decompress_device<hs::HeliosC>(cx, cx.status.as<uint32_t>(), n, cx.points.as<uint64_t>(), st, &nbad)
+            : decompress_device<hs::SeleneC>(cx, cx.status.as<uint32_t>(), n, cx.points.as<uint64_t>(), st, &nbad);
+    if (r)
+        return r;
+    HS_CHECK(hipStreamSynchronize(st));
+    if (nbad)
+        return fail(HS_GPU_EDECODE, std::to_string(nbad) + " point(s) failed to decode");
+    if ((r = msm_device_any(curve, cx, cx.scalars.p, cx.points.p, n, flags, cx.out32.p, nullptr, st)))
+        return r;
+    HS_CHECK(hipMemcpyAsync(out32, cx.out32.p, 32, hipMemcpyDeviceToHost, st));
+    HS_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
 
 const char *hs_gpu_version(void) { return "hs_gpu 0.1 gfx950"; }
 

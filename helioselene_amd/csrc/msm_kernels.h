@@ -32,54 +32,75 @@ struct SeleneC
 };
 
 // ---------------------------------------------------------------- memory helpers
-__device__ __forceinline__ void load_aff(aff &a, const aff *base, uint32_t idx)
+// Affine bases live in HBM as canonical little-endian words, x then y (64 bytes per point);
+// xyzz points (buckets, partial sums) as raw limbs, 4 x 9 words (144 bytes = 9 x 16 bytes).
+__device__ __forceinline__ void load_aff(aff &a, const uint32_t *base, uint32_t idx)
 {
-    const uint4 *p = reinterpret_cast<const uint4 *>(base + idx);
+    const uint4 *p = reinterpret_cast<const uint4 *>(base + 16ull * idx);
     uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
-    a.x.v[0] = q0.x; a.x.v[1] = q0.y; a.x.v[2] = q0.z; a.x.v[3] = q0.w;
-    a.x.v[4] = q1.x; a.x.v[5] = q1.y; a.x.v[6] = q1.z; a.x.v[7] = q1.w;
-    a.y.v[0] = q2.x; a.y.v[1] = q2.y; a.y.v[2] = q2.z; a.y.v[3] = q2.w;
-    a.y.v[4] = q3.x; a.y.v[5] = q3.y; a.y.v[6] = q3.z; a.y.v[7] = q3.w;
+    const uint32_t xw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    const uint32_t yw[8] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    fe_from_words(a.x, xw);
+    fe_from_words(a.y, yw);
 }
 
-__device__ __forceinline__ void store_fe(uint32_t *p, const fe &a)
+template <class F> __device__ __forceinline__ void store_aff(uint32_t *base, uint32_t idx, const aff &a)
 {
-    uint4 *q = reinterpret_cast<uint4 *>(p);
-    q[0] = make_uint4(a.v[0], a.v[1], a.v[2], a.v[3]);
-    q[1] = make_uint4(a.v[4], a.v[5], a.v[6], a.v[7]);
-}
-
-__device__ __forceinline__ void load_fe(fe &a, const uint32_t *p)
-{
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
-    uint4 q0 = q[0], q1 = q[1];
-    a.v[0] = q0.x; a.v[1] = q0.y; a.v[2] = q0.z; a.v[3] = q0.w;
-    a.v[4] = q1.x; a.v[5] = q1.y; a.v[6] = q1.z; a.v[7] = q1.w;
+    uint32_t xw[8], yw[8];
+    fe_to_words_canon<F>(xw, a.x);
+    fe_to_words_canon<F>(yw, a.y);
+    uint4 *p = reinterpret_cast<uint4 *>(base + 16ull * idx);
+    p[0] = make_uint4(xw[0], xw[1], xw[2], xw[3]);
+    p[1] = make_uint4(xw[4], xw[5], xw[6], xw[7]);
+    p[2] = make_uint4(yw[0], yw[1], yw[2], yw[3]);
+    p[3] = make_uint4(yw[4], yw[5], yw[6], yw[7]);
 }
 
 __device__ __forceinline__ void store_xyzz(xyzz *dst, const xyzz &p)
 {
-    uint32_t *b = reinterpret_cast<uint32_t *>(dst);
-    store_fe(b, p.X);
-    store_fe(b + 8, p.Y);
-    store_fe(b + 16, p.ZZ);
-    store_fe(b + 24, p.ZZZ);
+    uint32_t w[4 * NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        w[i] = p.X.v[i];
+        w[NL + i] = p.Y.v[i];
+        w[2 * NL + i] = p.ZZ.v[i];
+        w[3 * NL + i] = p.ZZZ.v[i];
+    }
+    uint4 *q = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
 __device__ __forceinline__ void load_xyzz(xyzz &p, const xyzz *src)
 {
-    const uint32_t *b = reinterpret_cast<const uint32_t *>(src);
-    load_fe(p.X, b);
-    load_fe(p.Y, b + 8);
-    load_fe(p.ZZ, b + 16);
-    load_fe(p.ZZZ, b + 24);
+    uint32_t w[4 * NL];
+    const uint4 *q = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        uint4 t = q[i];
+        w[4 * i] = t.x;
+        w[4 * i + 1] = t.y;
+        w[4 * i + 2] = t.z;
+        w[4 * i + 3] = t.w;
+    }
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        p.X.v[i] = w[i];
+        p.Y.v[i] = w[NL + i];
+        p.ZZ.v[i] = w[2 * NL + i];
+        p.ZZZ.v[i] = w[3 * NL + i];
+    }
 }
 
 // ---------------------------------------------------------------- ingest
 // Reference points: helios_jacobian = {X[5], Y[5], Z[5]} radix-2^51 uint64 limbs, Z = 0 is the
 // identity (helios/include/helios_ops.h:44-64).  Output canonical affine (X/Z^2, Y/Z^3).
 template <class C>
-__global__ void __launch_bounds__(256) k_ingest(const uint64_t *__restrict__ jac, uint32_t n, aff *__restrict__ out,
+__global__ void __launch_bounds__(256) k_ingest(const uint64_t *__restrict__ jac, uint32_t n, uint32_t *__restrict__ out,
                                                 uint8_t *__restrict__ is_id)
 {
     using F = typename C::F;
@@ -102,15 +123,15 @@ __global__ void __launch_bounds__(256) k_ingest(const uint64_t *__restrict__ jac
         l[k] = p[10 + k];
     fe_from_r51<F>(Z, l);
     aff a;
-    fe Zc;
-    fe_canon<F>(Zc, Z);
-    uint32_t zor = 0, one = Zc.v[0] == 1u;
+    uint32_t Zc[8];
+    fe_to_words_canon<F>(Zc, Z);
+    uint32_t zor = 0, one = Zc[0] == 1u;
 #pragma unroll
     for (int k = 0; k < 8; k++)
-        zor |= Zc.v[k];
+        zor |= Zc[k];
 #pragma unroll
     for (int k = 1; k < 8; k++)
-        one &= Zc.v[k] == 0u;
+        one &= Zc[k] == 0u;
     if (zor == 0)
     {
         fe_zero(a.x);
@@ -121,25 +142,21 @@ __global__ void __launch_bounds__(256) k_ingest(const uint64_t *__restrict__ jac
     {
         if (one)
         {
-            fe_canon<F>(a.x, X);
-            fe_canon<F>(a.y, Y);
+            a.x = X;
+            a.y = Y;
         }
         else
         {
-            fe zi, zi2, zi3, t;
+            fe zi, zi2, zi3;
             fe_inv<F>(zi, Z);
             fe_sq<F>(zi2, zi);
             fe_mul<F>(zi3, zi2, zi);
-            fe_mul<F>(t, X, zi2);
-            fe_canon<F>(a.x, t);
-            fe_mul<F>(t, Y, zi3);
-            fe_canon<F>(a.y, t);
+            fe_mul<F>(a.x, X, zi2);
+            fe_mul<F>(a.y, Y, zi3);
         }
         is_id[i] = 0;
     }
-    uint32_t *o = reinterpret_cast<uint32_t *>(out + i);
-    store_fe(o, a.x);
-    store_fe(o + 8, a.y);
+    store_aff<F>(out, i, a);
 }
 
 // ---------------------------------------------------------------- recode + histogram
@@ -157,7 +174,7 @@ __global__ void __launch_bounds__(256) k_recode(const uint32_t *__restrict__ sca
     uint4 a = sp[0], b = sp[1];
     s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
     s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
-    fe k;
+    uint32_t k[8];
     scalar_prepare<typename C::O>(k, s, w8_compat != 0);
     const bool skip = is_id[i] != 0;
     const uint32_t B = 1u << (c - 1);
@@ -272,7 +289,7 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ di
 
 // ---------------------------------------------------------------- bucket sums
 template <class C>
-__device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const aff *__restrict__ pts, uint32_t e)
+__device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const uint32_t *__restrict__ pts, uint32_t e)
 {
     using F = typename C::F;
     aff p;
@@ -285,7 +302,7 @@ __device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const aff *__
 // one thread per bucket with at most `lmax` entries; larger buckets are queued for k_bucket_big
 template <class C>
 __global__ void __launch_bounds__(256) k_bucket_small(const uint32_t *__restrict__ offsets, uint32_t nbuckets,
-                                                      const uint32_t *__restrict__ sorted, const aff *__restrict__ pts,
+                                                      const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ pts,
                                                       uint32_t lmax, xyzz *__restrict__ buckets,
                                                       uint32_t *__restrict__ big_list, uint32_t *__restrict__ big_count)
 {
@@ -339,7 +356,7 @@ template <class C> __device__ void block_tree_sum(xyzz &acc, xyzz *sh)
 template <class C>
 __global__ void __launch_bounds__(BIG_THREADS) k_bucket_big(const uint32_t *__restrict__ offsets,
                                                             const uint32_t *__restrict__ sorted,
-                                                            const aff *__restrict__ pts,
+                                                            const uint32_t *__restrict__ pts,
                                                             const uint32_t *__restrict__ big_list,
                                                             const uint32_t *__restrict__ big_count,
                                                             xyzz *__restrict__ buckets)
@@ -440,7 +457,7 @@ __global__ void k_combine(const xyzz *__restrict__ win_out, int W, int c, uint32
     aff a;
     bool pt = xyzz_to_aff<F>(a, acc);
     uint32_t enc[8];
-    encode_aff(enc, a, pt);
+    encode_aff<F>(enc, a, pt);
     if (out32)
         for (int k = 0; k < 8; k++)
             out32[k] = enc[k];
@@ -466,4 +483,134 @@ __global__ void k_combine(const xyzz *__restrict__ win_out, int W, int c, uint32
     }
 }
 
+} // namespace hs
+
+#include "encode.h"
+
+namespace hs
+{
+
+template <class C> struct CurveK;
+template <> struct CurveK<HeliosC>
+{
+    using K = HeliosK;
+};
+template <> struct CurveK<SeleneC>
+{
+    using K = SeleneK;
+};
+
+template <class F> __device__ __forceinline__ void store_r51_aff(uint64_t *o, const aff &a, bool is_point)
+{
+    uint64_t l[5];
+    if (is_point)
+    {
+        fe_to_r51<F>(l, a.x);
+        for (int k = 0; k < 5; k++)
+            o[k] = l[k];
+        fe_to_r51<F>(l, a.y);
+        for (int k = 0; k < 5; k++)
+            o[5 + k] = l[k];
+        for (int k = 0; k < 5; k++)
+            o[10 + k] = k == 0 ? 1u : 0u;
+    }
+    else
+    {
+        for (int k = 0; k < 15; k++)
+            o[k] = (k == 0 || k == 5) ? 1u : 0u;
+    }
+}
+
+// SSWU hash-to-curve of n 32-byte inputs -> reference Jacobian structs (Z = 1) and/or compressed
+template <class C>
+__global__ void __launch_bounds__(256) k_map_to_curve(const uint32_t *__restrict__ u, uint32_t n,
+                                                      uint64_t *__restrict__ out_jac, uint32_t *__restrict__ out32)
+{
+    using F = typename C::F;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t w[8];
+    for (int k = 0; k < 8; k++)
+        w[k] = u[8ull * i + k];
+    aff a;
+    sswu<F, typename CurveK<C>::K>(a, w);
+    if (out_jac)
+        store_r51_aff<F>(out_jac + 15ull * i, a, true);
+    if (out32)
+    {
+        uint32_t e[8];
+        encode_aff<F>(e, a, true);
+        for (int k = 0; k < 8; k++)
+            out32[8ull * i + k] = e[k];
+    }
+}
+
+// compressed points -> reference Jacobian structs; status[i] = 0 ok / 1 identity / 0xff invalid
+template <class C>
+__global__ void __launch_bounds__(256) k_decompress(const uint32_t *__restrict__ in32, uint32_t n,
+                                                    uint64_t *__restrict__ out_jac, uint8_t *__restrict__ status,
+                                                    uint32_t *__restrict__ nbad)
+{
+    using F = typename C::F;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t w[8];
+    for (int k = 0; k < 8; k++)
+        w[k] = in32[8ull * i + k];
+    aff a;
+    int rc = decompress<F, typename CurveK<C>::K>(a, w);
+    store_r51_aff<F>(out_jac + 15ull * i, a, rc == 0);
+    if (status)
+        status[i] = rc < 0 ? 0xffu : (uint8_t)rc;
+    if (rc < 0)
+        atomicAdd(nbad, 1u);
+}
+
+// INT32 multiply throughput probe: 8 independent v_mad_u64_u32 accumulation chains per thread
+// (one 32x32->64 partial product plus a 64-bit add per instruction, no other work in the loop).
+__global__ void __launch_bounds__(256) k_probe_mad64(uint32_t *out, int iters, uint32_t seed)
+{
+    uint64_t acc[8];
+    uint32_t a[8];
+    for (int j = 0; j < 8; j++)
+    {
+        acc[j] = threadIdx.x + j;
+        a[j] = seed * (j + 3) + threadIdx.x;
+    }
+    const uint32_t y = seed | 0x80000001u;
+    for (int i = 0; i < iters; i++)
+    {
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            asm volatile("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc[j]) : "v"(a[j]), "v"(y) : "vcc");
+    }
+    uint32_t r = 0;
+    for (int j = 0; j < 8; j++)
+        r ^= (uint32_t)(acc[j] >> 32) ^ (uint32_t)acc[j];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = r;
+}
+
+} // namespace hs
+
+namespace hs
+{
+// Encodes an array of xyzz points to compressed bytes (inspection of intermediate state).
+template <class C>
+__global__ void __launch_bounds__(64) k_encode_xyzz(const xyzz *__restrict__ p, uint32_t n, uint32_t *__restrict__ out32)
+{
+    using F = typename C::F;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    xyzz q;
+    load_xyzz(q, p + i);
+    aff a;
+    bool pt = xyzz_to_aff<F>(a, q);
+    uint32_t e[8];
+    encode_aff<F>(e, a, pt);
+    for (int k = 0; k < 8; k++)
+        out32[8ull * i + k] = e[k];
+}
 } // namespace hs

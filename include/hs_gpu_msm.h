@@ -83,6 +83,35 @@ int hs_gpu_msm_device(int curve, const void *d_scalars, const void *d_points, si
 /* Waits for all work the library queued on its own stream(s). */
 int hs_gpu_synchronize(void);
 
+/* Bytes form: points as 32-byte compressed encodings, decoded on the device like the
+ * reference's helios_frombytes / selene_frombytes (helios/src/x64/helios_frombytes.cpp:59-109,
+ * selene_frombytes.cpp:52-111), except that 0^32 is read as the identity (the reference test
+ * loader's convention, src/tests.cpp:7479-7496).  Returns HS_GPU_EDECODE if any point is invalid. */
+int hs_gpu_msm_bytes(int curve, unsigned char out32[32], const unsigned char *scalars, const unsigned char *points32,
+                     size_t n, unsigned flags);
+
+/* ---------------------------------------------------------------- hash-to-curve
+ * Simplified SWU map of n 32-byte inputs (RFC 9380 6.6.2; reference helios_map_to_curve /
+ * selene_map_to_curve, helios/src/x64/helios_map_to_curve.cpp:184-189).  Outputs the reference
+ * Jacobian struct with Z = 1 and/or the compressed encoding; either output may be NULL. */
+int hs_gpu_map_to_curve(int curve, const unsigned char *u, size_t n, hs_gpu_jacobian *out_points,
+                        unsigned char *out32);
+int hs_gpu_map_to_curve_device(int curve, const void *d_u, size_t n, void *d_points, void *d_out32, void *stream);
+
+/* ---------------------------------------------------------------- measurement
+ * No reference counterpart.  hs_gpu_profile(1) makes the plain MSM record HIP events between its
+ * phases; hs_gpu_phase_times() returns the last MSM's phase durations (ms) in the order given by
+ * hs_gpu_phase_name(i).  hs_gpu_probe_int_mul_peak() measures the device's 32x32->64 multiply
+ * throughput (v_mad_u64_u32 issue rate, partial products per second). */
+int hs_gpu_profile(int on);
+int hs_gpu_phase_times(float *ms, int cap);
+const char *hs_gpu_phase_name(int i);
+int hs_gpu_probe_int_mul_peak(double *products_per_second);
+/* Copies intermediate state of the last plain MSM on this device (inspection in tests): which = 0
+ * window sums, 1 segment sums, 2 bucket sums (as 32-byte encodings), 3 sorted bucket entries,
+ * 4 bucket offsets (uint32).  *count receives the number of items written. */
+int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t *count);
+
 /* ---------------------------------------------------------------- library */
 int hs_gpu_init(void);              /* selects device 0 (or the current device); idempotent */
 const char *hs_gpu_last_error(void);

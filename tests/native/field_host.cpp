@@ -3,22 +3,40 @@
 // without a GPU.  Built by tests/native/Makefile; never part of the product library.
 #include <stdint.h>
 #include <string.h>
-#define HS_HD static inline
+#define HS_HD inline
 #include "../../helioselene_amd/csrc/field.h"
 
 using namespace hs;
 
-static void ld(fe &r, const uint8_t *b)
+static void ldw(uint32_t w[8], const uint8_t *b)
 {
     for (int i = 0; i < 8; i++)
-        r.v[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) |
-                 ((uint32_t)b[4 * i + 3] << 24);
+        w[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) |
+               ((uint32_t)b[4 * i + 3] << 24);
 }
-static void st(uint8_t *b, const fe &a)
+static void stw(uint8_t *b, const uint32_t w[8])
 {
     for (int i = 0; i < 8; i++)
         for (int j = 0; j < 4; j++)
-            b[4 * i + j] = (uint8_t)(a.v[i] >> (8 * j));
+            b[4 * i + j] = (uint8_t)(w[i] >> (8 * j));
+}
+// 32 bytes (any value < 2^256) -> limbs
+static void ld(fe &r, const uint8_t *b)
+{
+    uint32_t w[8];
+    ldw(w, b);
+    fe_from_words(r, w);
+}
+// limbs -> canonical 32 bytes (field k decides the modulus)
+static int g_field = 0;
+static void st(uint8_t *b, const fe &a)
+{
+    uint32_t w[8];
+    if (g_field == 0)
+        fe_to_words_canon<Fp>(w, a);
+    else
+        fe_to_words_canon<Fq>(w, a);
+    stw(b, w);
 }
 
 template <class F> static void op(int o, uint8_t *out, const uint8_t *pa, const uint8_t *pb)
@@ -36,13 +54,59 @@ template <class F> static void op(int o, uint8_t *out, const uint8_t *pa, const 
     case 5: fe_canon<F>(r, a); break;
     case 6: fe_neg<F>(r, a); break;
     case 7: fe_mul_small<F>(r, a, pb[0] | (pb[1] << 8)); break;
-    default: memset(&r, 0, sizeof r);
+    default: fe_zero(r);
     }
     st(out, r);
 }
 
+// raw (non-canonical) limbs -> value as 36 bytes (checks the < 2^261 + small bound)
+static void st_raw(uint8_t *b, const fe &a)
+{
+    unsigned __int128 acc = 0;
+    int bits = 0, o = 0;
+    memset(b, 0, 40);
+    for (int i = 0; i < NL; i++)
+    {
+        // add a.v[i] << (29 i) into the little-endian byte array
+        unsigned __int128 x = (unsigned __int128)a.v[i] << ((29 * i) % 8);
+        int byte = (29 * i) / 8;
+        unsigned carry = 0;
+        for (int k = byte; k < 40; k++)
+        {
+            unsigned v = b[k] + (unsigned)(x & 0xff) + carry;
+            b[k] = (uint8_t)v;
+            carry = v >> 8;
+            x >>= 8;
+            if (!x && !carry)
+                break;
+        }
+    }
+    (void)acc; (void)bits; (void)o;
+}
+
+extern "C" void hst_fe_op_raw(int field, int o, uint8_t *out40, const uint8_t *pa, const uint8_t *pb)
+{
+    fe a, b, r;
+    ld(a, pa);
+    ld(b, pb);
+    if (field == 0)
+    {
+        switch (o) { case 0: fe_mul<Fp>(r, a, b); break; case 1: fe_sq<Fp>(r, a); break; case 2: fe_add<Fp>(r, a, b); break; default: fe_sub<Fp>(r, a, b); }
+    }
+    else
+    {
+        switch (o) { case 0: fe_mul<Fq>(r, a, b); break; case 1: fe_sq<Fq>(r, a); break; case 2: fe_add<Fq>(r, a, b); break; default: fe_sub<Fq>(r, a, b); }
+    }
+    st_raw(out40, r);
+    uint32_t mx = 0;
+    for (int i = 0; i < NL; i++)
+        mx = r.v[i] > mx ? r.v[i] : mx;
+    out40[39] = mx > (1u << 29) ? 1 : 0; // limb bound violated
+}
+
 extern "C" void hst_fe_op(int field, int o, uint8_t *out, const uint8_t *a, const uint8_t *b)
 {
+    g_field = field;
     if (field == 0)
         op<Fp>(o, out, a, b);
     else
@@ -51,6 +115,7 @@ extern "C" void hst_fe_op(int field, int o, uint8_t *out, const uint8_t *a, cons
 
 extern "C" void hst_fe_from_r51(int field, uint8_t *out, const uint64_t *l)
 {
+    g_field = field;
     fe r;
     if (field == 0)
         fe_from_r51<Fp>(r, l);
@@ -101,8 +166,90 @@ template <class F> static void pop(int o, uint8_t *out, const uint8_t *pa, const
 
 extern "C" void hst_pt_op(int field, int o, uint8_t *out, const uint8_t *a, const uint8_t *b, uint32_t k)
 {
+    g_field = field;
     if (field == 0)
         pop<Fp>(o, out, a, b, k);
     else
         pop<Fq>(o, out, a, b, k);
+}
+
+#include "../../helioselene_amd/csrc/encode.h"
+
+
+// compressed -> affine (x || y); returns 0 ok, 1 identity, -1 invalid
+extern "C" int hst_decompress(int curve, uint8_t *out, const uint8_t *in)
+{
+    g_field = curve;
+    uint32_t w[8];
+    ldw(w, in);
+    aff a;
+    int rc = curve == 0 ? decompress<Fp, HeliosK>(a, w) : decompress<Fq, SeleneK>(a, w);
+    st(out, a.x);
+    st(out + 32, a.y);
+    return rc;
+}
+
+extern "C" void hst_sswu(int curve, uint8_t *out, const uint8_t *u)
+{
+    g_field = curve;
+    uint32_t w[8];
+    ldw(w, u);
+    aff a;
+    if (curve == 0)
+        sswu<Fp, HeliosK>(a, w);
+    else
+        sswu<Fq, SeleneK>(a, w);
+    st(out, a.x);
+    st(out + 32, a.y);
+}
+
+// k doublings of an affine point, then + q; exercises xyzz_dbl / xyzz_add with ZZ != 1
+extern "C" void hst_dbl_chain(int field, uint8_t *out, const uint8_t *pa, const uint8_t *pb, int k)
+{
+    g_field = field;
+    aff a, b, r;
+    ld(a.x, pa); ld(a.y, pa + 32); ld(b.x, pb); ld(b.y, pb + 32);
+    bool ok;
+    if (field == 0)
+    {
+        xyzz A, B; xyzz_from_aff<Fp>(A, a); xyzz_from_aff<Fp>(B, b);
+        for (int i = 0; i < k; i++) xyzz_dbl<Fp>(A, A);
+        xyzz_dbl<Fp>(B, B);
+        xyzz_add<Fp>(A, A, B);
+        ok = xyzz_to_aff<Fp>(r, A);
+    }
+    else
+    {
+        xyzz A, B; xyzz_from_aff<Fq>(A, a); xyzz_from_aff<Fq>(B, b);
+        for (int i = 0; i < k; i++) xyzz_dbl<Fq>(A, A);
+        xyzz_dbl<Fq>(B, B);
+        xyzz_add<Fq>(A, A, B);
+        ok = xyzz_to_aff<Fq>(r, A);
+    }
+    st(out, r.x); st(out + 32, r.y); out[64] = ok;
+}
+
+#include "../../helioselene_amd/csrc/scalar.h"
+
+// prepared (reduced, optionally window-8 adjusted) scalar and its width-c digits (int32, signed)
+extern "C" int hst_recode(int curve, int c, int compat, const uint8_t *s32, uint8_t *prepared, int32_t *digits)
+{
+    uint32_t w[8];
+    ldw(w, s32);
+    uint32_t k[8];
+    if (curve == 0)
+        scalar_prepare<Fq>(k, w, compat != 0);
+    else
+        scalar_prepare<Fp>(k, w, compat != 0);
+    stw(prepared, k);
+    const int W = (256 + c - 1) / c;
+    uint32_t carry = 0;
+    for (int i = 0; i < W; i++)
+    {
+        uint32_t d = recode_digit(k, c, i, carry);
+        digits[i] = (d >> 31) ? -(int32_t)(d & 0x7fffffffu) : (int32_t)d;
+        if ((d >> 31) && (d & 0x7fffffffu) == 0)
+            return -1; // a signed zero must never be produced
+    }
+    return W;
 }
