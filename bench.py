@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain variable-time Pippenger MSM on one MI355X.
+
+One step = one Helios MSM and one Selene MSM of n = 2^20 points each (the single-MSM GPU
+workload of the Helios/Selene MSM path), on inputs already resident in HBM:
+  * bases: SSWU hash-to-curve images of seeded uniform 32-byte strings (computed on the GPU by
+    the library's own map), with 0.5% of them duplicates of another base and 0.1% the identity;
+  * scalars: uniform modulo each curve's group order, 1% forced to edge values
+    (0, 1, order - 1, a small 8-bit value).
+Prints one JSON line.  `value` is points per second over the timed steps (2 * n points per
+step).  `roofline` reports the bucket-accumulation kernel (the dominant one) against the
+device's 32x32->64 multiply throughput, measured in the same run by the library's probe;
+`cpu_baseline` times the reference's own CPU MSM (compiled from its sources into
+oracle/_ref/libhsref.so) on a bounded sample of the same inputs.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MSM points/s at n=2^20 (plain vs batched) + p50/p99 latency, bit-exact"
+# algorithmic partial products per field operation (8 x 32-bit limb schoolbook + reduction):
+# F_p mul 72, sq 44; F_q mul 112, sq 84; a batched-affine bucket addition is 5M + 1S
+PRODUCTS = {0: (72, 44), 1: (112, 84)}
+
+
+def algorithmic_bucket_ops(curve, n):
+    """INT32 multiply operations (2 per 32x32->64 partial product) of the bucket-accumulation
+    phase of an n-point MSM under the fixed convention: W(c*) * n bucket additions at 5M + 1S,
+    c* minimising the whole-MSM count."""
+    m, s = PRODUCTS[curve]
+    best = None
+    for c in range(4, 21):
+        W = -(-256 // c)
+        tot = W * n * (5 * m + s) + W * (1 << c) * (11 * m + 5 * s)
+        if best is None or tot < best[0]:
+            best = (tot, c, W)
+    _, c, W = best
+    return 2 * W * n * (5 * m + s), c
+
+
+def make_inputs(hs, torch, curve, n, seed, dev):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    order = hs.ORDER[curve]
+    u = torch.from_numpy(rng.integers(0, 256, size=(n, 32), dtype=np.uint8)).to(dev)
+    pts = torch.empty((n, 15), dtype=torch.int64, device=dev)
+    hs.map_to_curve_device(curve, u.data_ptr(), n, pts.data_ptr(), None, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ndup, nid = n // 200, n // 1000
+    dst = torch.from_numpy(rng.choice(n, size=ndup + nid, replace=False)).to(dev)
+    src = torch.from_numpy(rng.integers(0, n, size=ndup)).to(dev)
+    pts[dst[:ndup]] = pts[src]
+    ident = torch.zeros(15, dtype=torch.int64, device=dev)
+    ident[0] = 1
+    ident[5] = 1
+    pts[dst[ndup:]] = ident
+    # scalars: uniform in [0, 2^255); values >= order (probability ~2^-128) are reduced
+    sc = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+    sc[:, 31] &= 0x7F
+    top = sc[:, 16:].view(np.uint64)
+    for i in np.nonzero((top == np.uint64(0xFFFFFFFFFFFFFFFF)).all(axis=1))[0]:
+        v = int.from_bytes(sc[i].tobytes(), "little") % order
+        sc[i] = np.frombuffer(v.to_bytes(32, "little"), dtype=np.uint8)
+    k = n // 100
+    for j, i in enumerate(rng.choice(n, size=k, replace=False)):
+        v = (0, 1, order - 1, int(rng.integers(2, 256)))[j % 4]
+        sc[i] = np.frombuffer(v.to_bytes(32, "little"), dtype=np.uint8)
+    return torch.from_numpy(sc).to(dev), pts
+
+
+def cpu_baseline(hs_pts_cpu, sc_cpu, n_sample):
+    """The reference CPU MSM (x64 Pippenger behind its runtime dispatch) on the first n_sample
+    inputs of each curve, one thread."""
+    lib = os.path.join(ROOT, "oracle", "_ref", "libhsref.so")
+    if not os.path.exists(lib):
+        return None
+    L = ctypes.CDLL(lib)
+    L.hsref_init.restype = ctypes.c_uint
+    feats = L.hsref_init()
+    tot = 0.0
+    for curve in (0, 1):
+        f = getattr(L, ("hsref_helios_time_msm", "hsref_selene_time_msm")[curve])
+        f.restype = ctypes.c_double
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+        s = sc_cpu[curve][:n_sample].copy()
+        p = hs_pts_cpu[curve][:n_sample].copy()
+        tot += f(s.ctypes.data, p.ctypes.data, n_sample, 1, 1)
+    return {
+        "value": 2 * n_sample / tot,
+        "unit": "points/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": (f"one Helios and one Selene reference MSM (helios_msm_vartime / selene_msm_vartime via "
+                   f"helioselene_init dispatch, cpu feature mask {feats:#x}) at n={n_sample} on the first "
+                   f"{n_sample} bases/scalars of this workload, 1 thread, {tot:.1f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
+                    help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (each MSM fits one "
+              f"MI355X; multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import helioselene_amd as hs
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    streams = [torch.cuda.Stream() for _ in range(args.streams)]
+    n = args.n
+    with torch.cuda.stream(streams[0]):
+        inputs = [make_inputs(hs, torch, c, n, args.seed + 17 * c, dev) for c in (0, 1)]
+        out = torch.zeros((2, 8), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        sts = [streams[c % args.streams].cuda_stream for c in (0, 1)]
+
+        def step():
+            for c in (0, 1):
+                sc, pts = inputs[c]
+                hs.msm_device(c, sc.data_ptr(), pts.data_ptr(), n, out[c].data_ptr(), None, 0, sts[c])
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        hs.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        phases = [hs.phase_summary(c) for c in (0, 1)]
+        hs.profile(False)
+
+    elapsed = t1 - t0
+    value = args.steps * 2 * n / elapsed
+    peak_products = hs.probe_int_mul_peak()
+    ops, cstar = zip(*[algorithmic_bucket_ops(c, n) for c in (0, 1)])
+    bucket_ms = [phases[c][0].get("bucket", 0.0) / max(1, phases[c][1]) for c in (0, 1)]
+    achieved = (ops[0] + ops[1]) / ((bucket_ms[0] + bucket_ms[1]) * 1e-3)
+    peak = 2 * peak_products
+    per_curve_ms = {["helios", "selene"][c]: {k: v / max(1, phases[c][1]) for k, v in phases[c][0].items()}
+                    for c in (0, 1)}
+
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded SSWU bases with 0.5% duplicates and 0.1% identities; scalars uniform mod order "
+                "with 1% edge values)",
+        "config": {
+            "workload": "plain single MSM, one Helios and one Selene MSM of n=2^20 points per step",
+            "n": n,
+            "curves": ["helios", "selene"],
+            "window_bits": [hs.window_bits(c, n) for c in (0, 1)],
+            "result_mode": "reference-compatible",
+            "streams": args.streams,
+        },
+        "roofline": {
+            "bound": "valu_int32",
+            "kernel": "k_bucket (bucket accumulation)",
+            "achieved": achieved / 1e12,
+            "peak": peak / 1e12,
+            "unit": "TOP/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "algorithmic_ops_per_launch": {"helios": ops[0], "selene": ops[1], "c_star": list(cstar)},
+            "kernel_ms": {"helios": bucket_ms[0], "selene": bucket_ms[1]},
+            "peak_source": "measured v_mad_u64_u32 throughput x 2 INT32 ops per partial product",
+        },
+        "phase_ms": per_curve_ms,
+        "per_curve_points_per_s": {["helios", "selene"][c]: n / (sum(per_curve_ms[["helios", "selene"][c]].values()) * 1e-3)
+                                   for c in (0, 1)},
+    }
+    if not args.no_cpu:
+        pts_cpu = [inputs[c][1].cpu().numpy().view(np.uint64) for c in (0, 1)]
+        sc_cpu = [inputs[c][0].cpu().numpy() for c in (0, 1)]
+        rec["cpu_baseline"] = cpu_baseline(pts_cpu, sc_cpu, min(n, args.cpu_sample))
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -55,6 +55,8 @@ def lib():
         L.hs_gpu_profile.argtypes = [ctypes.c_int]
         L.hs_gpu_phase_times.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         L.hs_gpu_phase_name.restype = cp
+        L.hs_gpu_phase_summary.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int)]
         L.hs_gpu_probe_int_mul_peak.argtypes = [ctypes.POINTER(ctypes.c_double)]
         L.hs_gpu_debug_dump.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
@@ -226,3 +228,14 @@ def debug_dump(curve, which, max_items):
     if which < 3:
         return [buf[32 * i:32 * i + 32].tobytes() for i in range(cnt.value)]
     return buf.view(np.uint32)[:cnt.value].copy()
+
+
+def phase_summary(curve=-1):
+    """({phase: total ms}, calls) over all MSMs profiled since profile(True) (curve -1: both)."""
+    buf = (ctypes.c_float * 16)()
+    calls = ctypes.c_int()
+    c = -1 if curve == -1 else CURVES[curve]
+    k = lib().hs_gpu_phase_summary(c, buf, 16, ctypes.byref(calls))
+    if k < 0:
+        _check(k)
+    return {lib().hs_gpu_phase_name(i).decode(): float(buf[i]) for i in range(k)}, calls.value

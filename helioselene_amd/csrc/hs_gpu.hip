@@ -108,24 +108,72 @@ int ref_window(size_t n)
     return 11;
 }
 
+// Scratch of one in-flight MSM; one workspace per stream, so MSMs issued on different streams
+// may overlap on the device.
+struct Workspace
+{
+    DevBuf aff, isid, digits, counts, offsets, block_sums, sorted, buckets, seg, win, big_list, big_count, entries,
+        cnt_a, off_a;
+};
+
 struct Ctx
 {
     int device = -1;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    DevBuf scalars, points, aff, isid, digits, counts, offsets, block_sums, sorted, buckets, seg, win, big_list,
-        big_count, out32, outjac, status, nbad, probe;
+    DevBuf scalars, points, out32, outjac, status, nbad, probe;
+    std::vector<std::pair<hipStream_t, Workspace *>> wss;
     bool ready = false;
-    // phase timing (hs_gpu_profile): events recorded between the phases of the last plain MSM
+
+    Workspace &ws(hipStream_t st)
+    {
+        for (auto &p : wss)
+            if (p.first == st)
+                return *p.second;
+        wss.emplace_back(st, new Workspace());
+        return *wss.back().second;
+    }
+    // phase timing (hs_gpu_profile): HIP events recorded between the phases of every plain MSM
+    // issued while profiling is on, on the stream the MSM runs on
     bool profile = false;
-    static constexpr int NEV = 8;
-    hipEvent_t ev[NEV] = {};
+    static constexpr int NEV = 7; // 6 phases
+    struct EvSet
+    {
+        hipEvent_t ev[NEV];
+        int curve;
+        int n;
+    };
+    std::vector<EvSet> evsets;
+    size_t evused = 0;
     int nev = 0;
+    EvSet *cur = nullptr;
+
+    int begin_profile(int curve)
+    {
+        cur = nullptr;
+        nev = 0;
+        if (!profile)
+            return 0;
+        if (evused == evsets.size())
+        {
+            EvSet e;
+            for (int i = 0; i < NEV; i++)
+                HS_CHECK(hipEventCreate(&e.ev[i]));
+            evsets.push_back(e);
+        }
+        cur = &evsets[evused++];
+        cur->curve = curve;
+        cur->n = 0;
+        return 0;
+    }
 
     void mark(hipStream_t st)
     {
-        if (profile && nev < NEV)
-            hipEventRecord(ev[nev++], st);
+        if (cur && nev < NEV)
+        {
+            hipEventRecord(cur->ev[nev++], st);
+            cur->n = nev;
+        }
     }
 
     int init()
@@ -140,8 +188,6 @@ struct Ctx
             return fail(HS_GPU_EDEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
         device = dev;
         HS_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (int i = 0; i < NEV; i++)
-            HS_CHECK(hipEventCreate(&ev[i]));
         ready = true;
         return 0;
     }
@@ -153,7 +199,7 @@ Ctx &ctx()
     return c;
 }
 
-int scan_exclusive(Ctx &cx, const uint32_t *in, uint32_t *out, uint32_t len, hipStream_t st)
+int scan_exclusive(Workspace &cx, const uint32_t *in, uint32_t *out, uint32_t len, hipStream_t st)
 {
     const uint32_t per = hs::SCAN_BLOCK * hs::SCAN_ITEMS;
     const uint32_t nblk = (len + per - 1) / per;
@@ -176,39 +222,54 @@ template <class C>
 int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, size_t n, unsigned flags, uint32_t *d_out32,
                uint64_t *d_outjac, hipStream_t st)
 {
-    if (n >= (1ull << 31))
-        return fail(HS_GPU_EINVAL, "n too large");
+    if (n >= (1ull << 24))
+        return fail(HS_GPU_EINVAL, "n must be below 2^24 points per MSM");
+    Workspace &ws = cx.ws(st);
     const Plan P = make_plan(n);
     const uint32_t nn = (uint32_t)n;
     const bool compat = !(flags & HS_GPU_EXACT) && ref_window(n) == 8;
+    // two-level sort geometry: nsb super-buckets of 2^shift buckets per window, blocks of T points
+    const int shift = P.c - 1 > 8 ? P.c - 1 - 8 : 0;
+    const int nsb = (int)(P.B >> shift);
+    uint32_t T = 4096;
+    while ((nn + T - 1) / T > 512)
+        T *= 2;
+    const uint32_t nblk = (nn + T - 1) / T;
+    const uint32_t nh = (uint32_t)P.W * nsb;
+    const size_t lds = (size_t)nh * 4;
     int r;
-    if ((r = cx.aff.reserve(n * 64)) || (r = cx.isid.reserve(n)) || (r = cx.digits.reserve(n * 4 * (size_t)P.W)) ||
-        (r = cx.counts.reserve((size_t)(P.nb + 1) * 4)) || (r = cx.offsets.reserve((size_t)(P.nb + 1) * 4)) ||
-        (r = cx.sorted.reserve(n * 4 * (size_t)P.W + 4)) || (r = cx.buckets.reserve((size_t)P.nb * sizeof(hs::xyzz))) ||
-        (r = cx.seg.reserve((size_t)P.segs * sizeof(hs::xyzz))) || (r = cx.win.reserve((size_t)P.W * sizeof(hs::xyzz))) ||
-        (r = cx.big_list.reserve((size_t)P.nb * 4)) || (r = cx.big_count.reserve(64)))
+    if ((r = ws.aff.reserve(n * 64)) || (r = ws.isid.reserve(n)) || (r = ws.digits.reserve(n * 2 * (size_t)P.W)) ||
+        (r = ws.offsets.reserve((size_t)(P.nb + 1) * 4)) || (r = ws.entries.reserve(n * 4 * (size_t)P.W + 4)) ||
+        (r = ws.cnt_a.reserve(((size_t)nh * nblk + 1) * 4)) || (r = ws.off_a.reserve(((size_t)nh * nblk + 1) * 4)) ||
+        (r = ws.sorted.reserve(n * 4 * (size_t)P.W + 4)) || (r = ws.buckets.reserve((size_t)P.nb * sizeof(hs::xyzz))) ||
+        (r = ws.seg.reserve((size_t)P.segs * sizeof(hs::xyzz))) || (r = ws.win.reserve((size_t)P.W * sizeof(hs::xyzz))) ||
+        (r = ws.big_list.reserve((size_t)P.nb * 4)) || (r = ws.big_count.reserve(64)))
         return r;
-    uint32_t *aff = cx.aff.as<uint32_t>();
-    uint8_t *isid = cx.isid.as<uint8_t>();
-    uint32_t *digits = cx.digits.as<uint32_t>(), *counts = cx.counts.as<uint32_t>(),
-             *offsets = cx.offsets.as<uint32_t>(), *sorted = cx.sorted.as<uint32_t>();
-    hs::xyzz *buckets = cx.buckets.as<hs::xyzz>(), *seg = cx.seg.as<hs::xyzz>(), *win = cx.win.as<hs::xyzz>();
-    uint32_t *big_list = cx.big_list.as<uint32_t>(), *big_count = cx.big_count.as<uint32_t>();
+    uint32_t *aff = ws.aff.as<uint32_t>();
+    uint8_t *isid = ws.isid.as<uint8_t>();
+    uint16_t *digits = ws.digits.as<uint16_t>();
+    uint32_t *offsets = ws.offsets.as<uint32_t>(), *sorted = ws.sorted.as<uint32_t>(), *entries = ws.entries.as<uint32_t>(),
+             *cnt_a = ws.cnt_a.as<uint32_t>(), *off_a = ws.off_a.as<uint32_t>();
+    hs::xyzz *buckets = ws.buckets.as<hs::xyzz>(), *seg = ws.seg.as<hs::xyzz>(), *win = ws.win.as<hs::xyzz>();
+    uint32_t *big_list = ws.big_list.as<uint32_t>(), *big_count = ws.big_count.as<uint32_t>();
 
     const dim3 blk(256), grd((nn + 255) / 256);
-    cx.nev = 0;
+    if ((r = cx.begin_profile(C::id)))
+        return r;
     cx.mark(st);
     hipLaunchKernelGGL(hs::k_ingest<C>, grd, blk, 0, st, d_points, nn, aff, isid);
     cx.mark(st);
-    HS_CHECK(hipMemsetAsync(counts, 0, (size_t)(P.nb + 1) * 4, st));
-    HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
-    hipLaunchKernelGGL(hs::k_recode<C>, grd, blk, 0, st, d_scalars, isid, nn, P.c, P.W, compat ? 1 : 0, digits, counts);
+    hipLaunchKernelGGL(hs::k_recode16<C>, grd, blk, 0, st, d_scalars, isid, nn, P.c, P.W, compat ? 1 : 0, digits);
     cx.mark(st);
-    if ((r = scan_exclusive(cx, counts, offsets, P.nb + 1, st)))
+    hipLaunchKernelGGL(hs::k_coarse_count, dim3(nblk), blk, lds, st, digits, nn, P.W, nsb, shift, T, nblk, cnt_a);
+    HS_CHECK(hipMemsetAsync(cnt_a + (size_t)nh * nblk, 0, 4, st));
+    if ((r = scan_exclusive(ws, cnt_a, off_a, nh * nblk + 1, st)))
         return r;
-    // cursor = offsets (counts reused as the cursor array)
-    HS_CHECK(hipMemcpyAsync(counts, offsets, (size_t)P.nb * 4, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(hs::k_scatter, grd, blk, 0, st, digits, nn, P.W, P.B, counts, sorted);
+    hipLaunchKernelGGL(hs::k_coarse_scatter, dim3(nblk), blk, lds, st, digits, nn, P.W, nsb, shift, T, nblk, off_a,
+                       entries);
+    hipLaunchKernelGGL(hs::k_fine_sort, dim3(nh), blk, 0, st, entries, off_a, nblk, nsb, shift, P.B, nh, sorted,
+                       offsets);
+    HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
     cx.mark(st);
     const uint32_t lmax = 96;
     hipLaunchKernelGGL(hs::k_bucket_small<C>, dim3((P.nb + 255) / 256), blk, 0, st, offsets, P.nb, sorted, aff, lmax,
@@ -386,20 +447,52 @@ int hs_gpu_profile(int on)
     if ((r = cx.init()))
         return r;
     cx.profile = on != 0;
-    cx.nev = 0;
+    cx.evused = 0;
+    cx.cur = nullptr;
     return 0;
+}
+
+int hs_gpu_phase_summary(int curve, float *ms, int cap, int *calls)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int k = 0, nc = 0;
+    for (int i = 0; i < cap; i++)
+        ms[i] = 0.f;
+    for (size_t s = 0; s < cx.evused; s++)
+    {
+        Ctx::EvSet &e = cx.evsets[s];
+        if ((curve >= 0 && e.curve != curve) || e.n < 2)
+            continue;
+        HS_CHECK(hipEventSynchronize(e.ev[e.n - 1]));
+        nc++;
+        for (int i = 0; i + 1 < e.n && i < cap; i++)
+        {
+            float t;
+            HS_CHECK(hipEventElapsedTime(&t, e.ev[i], e.ev[i + 1]));
+            ms[i] += t;
+            if (i + 1 > k)
+                k = i + 1;
+        }
+    }
+    if (calls)
+        *calls = nc;
+    return k;
 }
 
 int hs_gpu_phase_times(float *ms, int cap)
 {
     Ctx &cx = ctx();
-    std::lock_guard<std::mutex> lk(cx.mu);
-    if (!cx.ready || cx.nev < 2)
-        return 0;
-    HS_CHECK(hipEventSynchronize(cx.ev[cx.nev - 1]));
     int k = 0;
-    for (int i = 0; i + 1 < cx.nev && k < cap; i++, k++)
-        HS_CHECK(hipEventElapsedTime(&ms[k], cx.ev[i], cx.ev[i + 1]));
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if (cx.evused == 0)
+            return 0;
+        Ctx::EvSet &e = cx.evsets[cx.evused - 1];
+        HS_CHECK(hipEventSynchronize(e.ev[e.n - 1]));
+        for (int i = 0; i + 1 < e.n && i < cap; i++, k++)
+            HS_CHECK(hipEventElapsedTime(&ms[i], e.ev[i], e.ev[i + 1]));
+    }
     return k;
 }
 
@@ -415,16 +508,17 @@ int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t 
     if (!cx.ready)
         return fail(HS_GPU_EINVAL, "no MSM has run");
     *count = 0;
+    Workspace &ws = cx.ws(cx.stream);
     if (which == 3 || which == 4)
     {
         // 3: sorted entries, 4: bucket offsets (raw uint32)
-        DevBuf &b = which == 3 ? cx.sorted : cx.offsets;
+        DevBuf &b = which == 3 ? ws.sorted : ws.offsets;
         size_t nb = std::min(cap_bytes, b.cap);
         HS_CHECK(hipMemcpy(out, b.p, nb, hipMemcpyDeviceToHost));
         *count = nb / 4;
         return 0;
     }
-    DevBuf &src = which == 0 ? cx.win : which == 1 ? cx.seg : cx.buckets;
+    DevBuf &src = which == 0 ? ws.win : which == 1 ? ws.seg : ws.buckets;
     size_t n = std::min(cap_bytes / 32, src.cap / sizeof(hs::xyzz));
     int r;
     if ((r = cx.probe.reserve(n * 32 + 64)))
@@ -490,12 +584,12 @@ int hs_gpu_map_to_curve(int curve, const unsigned char *u, size_t n, hs_gpu_jaco
     std::lock_guard<std::mutex> lk(cx.mu);
     int r;
     if ((r = cx.init()) || (r = cx.scalars.reserve(n * 32)) || (r = cx.points.reserve(n * sizeof(hs_gpu_jacobian))) ||
-        (r = cx.digits.reserve(n * 32)))
+        (r = cx.status.reserve(n * 32)))
         return r;
     hipStream_t st = cx.stream;
     HS_CHECK(hipMemcpyAsync(cx.scalars.p, u, n * 32, hipMemcpyHostToDevice, st));
     uint64_t *dj = out_points ? cx.points.as<uint64_t>() : nullptr;
-    uint32_t *d32 = out32 ? cx.digits.as<uint32_t>() : nullptr;
+    uint32_t *d32 = out32 ? cx.status.as<uint32_t>() : nullptr;
     r = curve == HS_GPU_HELIOS ? map_to_curve_device<hs::HeliosC>(cx, cx.scalars.as<uint32_t>(), n, dj, d32, st)
                                : map_to_curve_device<hs::SeleneC>(cx, cx.scalars.as<uint32_t>(), n, dj, d32, st);
     if (r)

@@ -287,6 +287,136 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ di
     }
 }
 
+// ---------------------------------------------------------------- two-level counting sort
+// Digits are stored as 16-bit codes, window-major: code = (|d| - 1) | (d < 0) << 15, and
+// DIGIT_ZERO for a zero digit (that code would mean digit -2^15, which never occurs because
+// digits lie in [-(2^(c-1)) + 1, 2^(c-1)] with c <= 16).
+constexpr uint16_t DIGIT_ZERO = 0xffffu;
+
+template <class C>
+__global__ void __launch_bounds__(256) k_recode16(const uint32_t *__restrict__ scalars, const uint8_t *__restrict__ is_id,
+                                                  uint32_t n, int c, int W, int w8_compat, uint16_t *__restrict__ digits)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t s[8];
+    const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * i);
+    uint4 a = sp[0], b = sp[1];
+    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+    s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    uint32_t k[8];
+    scalar_prepare<typename C::O>(k, s, w8_compat != 0);
+    const bool skip = is_id[i] != 0;
+    uint32_t carry = 0;
+    for (int w = 0; w < W; w++)
+    {
+        uint32_t d = recode_digit(k, c, w, carry);
+        uint16_t code = (d == 0 || skip) ? DIGIT_ZERO : (uint16_t)(((d & 0x7fffffffu) - 1) | ((d >> 31) << 15));
+        digits[(size_t)w * n + i] = code;
+    }
+}
+
+// Coarse pass: block b takes points [b*T, b*T + T) of every window and counts them per
+// (window, super-bucket), super-bucket = bucket >> shift.  counts[(w*nsb + sb) * nblk + b].
+__global__ void __launch_bounds__(256) k_coarse_count(const uint16_t *__restrict__ digits, uint32_t n, int W,
+                                                      int nsb, int shift, uint32_t T, uint32_t nblk,
+                                                      uint32_t *__restrict__ counts)
+{
+    extern __shared__ uint32_t hist[];
+    const uint32_t b = blockIdx.x, beg = b * T, cnt = min(T, n - beg), nh = (uint32_t)W * nsb;
+    for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
+        hist[j] = 0;
+    __syncthreads();
+    const uint32_t tot = (uint32_t)W * cnt;
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x)
+    {
+        const uint32_t w = idx / cnt, i = beg + idx % cnt;
+        const uint16_t code = digits[(size_t)w * n + i];
+        if (code != DIGIT_ZERO)
+            atomicAdd(&hist[w * nsb + ((code & 0x7fffu) >> shift)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
+        counts[(size_t)j * nblk + b] = hist[j];
+}
+
+// Coarse scatter: entry = point index (bits 0..23) | sub-bucket << 24 | sign << 31
+__global__ void __launch_bounds__(256) k_coarse_scatter(const uint16_t *__restrict__ digits, uint32_t n, int W,
+                                                        int nsb, int shift, uint32_t T, uint32_t nblk,
+                                                        const uint32_t *__restrict__ offs,
+                                                        uint32_t *__restrict__ entries)
+{
+    extern __shared__ uint32_t cur[];
+    const uint32_t b = blockIdx.x, beg = b * T, cnt = min(T, n - beg), nh = (uint32_t)W * nsb;
+    for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
+        cur[j] = offs[(size_t)j * nblk + b];
+    __syncthreads();
+    const uint32_t tot = (uint32_t)W * cnt, smask = (1u << shift) - 1;
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x)
+    {
+        const uint32_t w = idx / cnt, i = beg + idx % cnt;
+        const uint16_t code = digits[(size_t)w * n + i];
+        if (code != DIGIT_ZERO)
+        {
+            const uint32_t mag1 = code & 0x7fffu;
+            const uint32_t pos = atomicAdd(&cur[w * nsb + (mag1 >> shift)], 1u);
+            entries[pos] = i | ((mag1 & smask) << 24) | ((uint32_t)(code >> 15) << 31);
+        }
+    }
+}
+
+// Fine pass: one block per (window, super-bucket) region; orders the region by bucket, writes
+// the final entries (point index | sign << 31) and the bucket offsets of the region.
+__global__ void __launch_bounds__(256) k_fine_sort(const uint32_t *__restrict__ entries, const uint32_t *__restrict__ offs,
+                                                   uint32_t nblk, int nsb, int shift, uint32_t B, uint32_t nregions,
+                                                   uint32_t *__restrict__ sorted, uint32_t *__restrict__ offsets)
+{
+    __shared__ uint32_t hist[128];
+    __shared__ uint32_t cur[128];
+    const uint32_t r = blockIdx.x;
+    const uint32_t start = offs[(size_t)r * nblk], end = offs[(size_t)(r + 1) * nblk];
+    const uint32_t nsub = 1u << shift;
+    for (uint32_t j = threadIdx.x; j < nsub; j += blockDim.x)
+        hist[j] = 0;
+    __syncthreads();
+    for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x)
+        atomicAdd(&hist[(entries[e] >> 24) & 0x7fu], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64)
+    {
+        // exclusive scan of up to 128 counters by one wave (two per lane)
+        const uint32_t l = threadIdx.x;
+        uint32_t a0 = 2 * l < nsub ? hist[2 * l] : 0u, a1 = 2 * l + 1 < nsub ? hist[2 * l + 1] : 0u;
+        uint32_t x = a0 + a1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            uint32_t y = __shfl_up(x, o, 64);
+            if ((int)l >= o)
+                x += y;
+        }
+        const uint32_t ex = x - a0 - a1;
+        if (2 * l < nsub)
+            cur[2 * l] = start + ex;
+        if (2 * l + 1 < nsub)
+            cur[2 * l + 1] = start + ex + a0;
+    }
+    __syncthreads();
+    const uint32_t w = r / nsb, sb = r % nsb;
+    for (uint32_t j = threadIdx.x; j < nsub; j += blockDim.x)
+        offsets[(size_t)w * B + sb * nsub + j] = cur[j];
+    if (r == nregions - 1 && threadIdx.x == 0)
+        offsets[(size_t)(w + 1) * B] = end;
+    __syncthreads();
+    for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x)
+    {
+        const uint32_t v = entries[e];
+        const uint32_t pos = atomicAdd(&cur[(v >> 24) & 0x7fu], 1u);
+        sorted[pos] = (v & 0x00ffffffu) | (v & 0x80000000u);
+    }
+}
+
 // ---------------------------------------------------------------- bucket sums
 template <class C>
 __device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const uint32_t *__restrict__ pts, uint32_t e)

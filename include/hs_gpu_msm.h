@@ -99,12 +99,15 @@ int hs_gpu_map_to_curve(int curve, const unsigned char *u, size_t n, hs_gpu_jaco
 int hs_gpu_map_to_curve_device(int curve, const void *d_u, size_t n, void *d_points, void *d_out32, void *stream);
 
 /* ---------------------------------------------------------------- measurement
- * No reference counterpart.  hs_gpu_profile(1) makes the plain MSM record HIP events between its
- * phases; hs_gpu_phase_times() returns the last MSM's phase durations (ms) in the order given by
- * hs_gpu_phase_name(i).  hs_gpu_probe_int_mul_peak() measures the device's 32x32->64 multiply
- * throughput (v_mad_u64_u32 issue rate, partial products per second). */
+ * No reference counterpart.  hs_gpu_profile(1) (re)starts profiling: every plain MSM then records
+ * HIP events between its phases on the stream it runs on.  hs_gpu_phase_times() returns the last
+ * MSM's phase durations (ms), hs_gpu_phase_summary() the per-phase totals over all profiled MSMs
+ * of one curve (curve = -1: both) and their count; phases are named by hs_gpu_phase_name(i).
+ * hs_gpu_probe_int_mul_peak() measures the device's 32x32->64 multiply throughput
+ * (v_mad_u64_u32 issue rate, partial products per second). */
 int hs_gpu_profile(int on);
 int hs_gpu_phase_times(float *ms, int cap);
+int hs_gpu_phase_summary(int curve, float *ms, int cap, int *calls);
 const char *hs_gpu_phase_name(int i);
 int hs_gpu_probe_int_mul_peak(double *products_per_second);
 /* Copies intermediate state of the last plain MSM on this device (inspection in tests): which = 0
