@@ -251,6 +251,144 @@ template <class F> HS_HD bool xyzz_to_aff(aff &r, const xyzz &p)
     return true;
 }
 
+// ---------------------------------------------------------------- Jacobian (x = X/Z^2, y = Y/Z^3)
+// Used for long doubling chains (window combine), where dbl-2001-b (3M + 5S, dependency depth
+// four) beats the xyzz doubling.
+struct jac
+{
+    fe X, Y, Z;
+};
+
+template <class F> HS_HD void jac_identity(jac &r)
+{
+    fe_one(r.X);
+    fe_one(r.Y);
+    fe_zero(r.Z);
+}
+
+// xyzz -> Jacobian without inversion: Z' = ZZ*ZZZ, X' = X*ZZ*ZZZ^2, Y' = Y*ZZ^3*ZZZ^2
+// (X'/Z'^2 = X/ZZ and Y'/Z'^3 = Y/ZZZ for any ZZ, ZZZ).
+template <class F> HS_HD void xyzz_to_jac(jac &r, const xyzz &p)
+{
+    fe zz2, zzz2, t;
+    fe_mul<F>(r.Z, p.ZZ, p.ZZZ);
+    fe_sq<F>(zzz2, p.ZZZ);
+    fe_mul<F>(t, p.X, p.ZZ);
+    fe_mul<F>(r.X, t, zzz2);
+    fe_sq<F>(zz2, p.ZZ);
+    fe_mul<F>(t, zz2, p.ZZ);
+    fe_mul<F>(t, t, zzz2);
+    fe_mul<F>(r.Y, p.Y, t);
+}
+
+// dbl-2001-b, a = -3 (reference helios/src/x64/helios_dbl.cpp:32-88); the identity (Z = 0)
+// maps to itself.
+template <class F> HS_HD void jac_dbl(jac &r, const jac &p)
+{
+    fe delta, gamma, beta, alpha, t0, t1, X3, Y3, Z3;
+    fe_sq<F>(delta, p.Z);
+    fe_sq<F>(gamma, p.Y);
+    fe_mul<F>(beta, p.X, gamma);
+    fe_sub<F>(t0, p.X, delta);
+    fe_add<F>(t1, p.X, delta);
+    fe_mul<F>(alpha, t0, t1);
+    fe_mul_small<F>(alpha, alpha, 3);
+    fe_sq<F>(X3, alpha);
+    fe_mul_small<F>(t0, beta, 8);
+    fe_sub<F>(X3, X3, t0);
+    fe_add<F>(t1, p.Y, p.Z);
+    fe_sq<F>(Z3, t1);
+    fe_sub<F>(Z3, Z3, gamma);
+    fe_sub<F>(Z3, Z3, delta);
+    fe_mul_small<F>(t0, beta, 4);
+    fe_sub<F>(t0, t0, X3);
+    fe_mul<F>(Y3, alpha, t0);
+    fe_sq<F>(t1, gamma);
+    fe_mul_small<F>(t1, t1, 8);
+    fe_sub<F>(Y3, Y3, t1);
+    r.X = X3;
+    r.Y = Y3;
+    r.Z = Z3;
+}
+
+// add-2007-bl with the identity / equal / opposite cases (helios_add.cpp:32-107,
+// helios/include/helios_add.h:44-87)
+template <class F> HS_HD void jac_add(jac &r, const jac &p, const jac &q)
+{
+    if (fe_is_zero<F>(p.Z))
+    {
+        r = q;
+        return;
+    }
+    if (fe_is_zero<F>(q.Z))
+    {
+        r = p;
+        return;
+    }
+    fe Z1Z1, Z2Z2, U1, U2, S1, S2, H, I, J, rr, V, t0, t1;
+    fe_sq<F>(Z1Z1, p.Z);
+    fe_sq<F>(Z2Z2, q.Z);
+    fe_mul<F>(U1, p.X, Z2Z2);
+    fe_mul<F>(U2, q.X, Z1Z1);
+    fe_mul<F>(t0, q.Z, Z2Z2);
+    fe_mul<F>(S1, p.Y, t0);
+    fe_mul<F>(t0, p.Z, Z1Z1);
+    fe_mul<F>(S2, q.Y, t0);
+    fe_sub<F>(H, U2, U1);
+    fe_sub<F>(rr, S2, S1);
+    if (fe_is_zero<F>(H))
+    {
+        if (fe_is_zero<F>(rr))
+            jac_dbl<F>(r, p);
+        else
+            jac_identity<F>(r);
+        return;
+    }
+    fe_add<F>(t0, H, H);
+    fe_sq<F>(I, t0);
+    fe_mul<F>(J, H, I);
+    fe_add<F>(rr, rr, rr);
+    fe_mul<F>(V, U1, I);
+    fe X3, Y3, Z3;
+    fe_sq<F>(X3, rr);
+    fe_sub<F>(X3, X3, J);
+    fe_add<F>(t0, V, V);
+    fe_sub<F>(X3, X3, t0);
+    fe_sub<F>(t0, V, X3);
+    fe_mul<F>(t1, rr, t0);
+    fe_mul<F>(t0, S1, J);
+    fe_add<F>(t0, t0, t0);
+    fe_sub<F>(Y3, t1, t0);
+    fe_add<F>(t0, p.Z, q.Z);
+    fe_sq<F>(t1, t0);
+    fe_sub<F>(t1, t1, Z1Z1);
+    fe_sub<F>(t1, t1, Z2Z2);
+    fe_mul<F>(Z3, t1, H);
+    r.X = X3;
+    r.Y = Y3;
+    r.Z = Z3;
+}
+
+// Jacobian -> affine (canonical); returns false for the identity
+template <class F> HS_HD bool jac_to_aff(aff &r, const jac &p)
+{
+    if (fe_is_zero<F>(p.Z))
+    {
+        fe_zero(r.x);
+        fe_zero(r.y);
+        return false;
+    }
+    fe zi, zi2, zi3, t;
+    fe_inv<F>(zi, p.Z);
+    fe_sq<F>(zi2, zi);
+    fe_mul<F>(zi3, zi2, zi);
+    fe_mul<F>(t, p.X, zi2);
+    fe_canon<F>(r.x, t);
+    fe_mul<F>(t, p.Y, zi3);
+    fe_canon<F>(r.y, t);
+    return true;
+}
+
 // 32-byte compressed encoding (reference helios_tobytes.cpp:42-66): canonical x little-endian,
 // bit 255 = parity of canonical y, identity = 32 zero bytes.
 template <class F> HS_HD void encode_aff(uint32_t out[8], const aff &a, bool is_point)

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -59,7 +60,7 @@ struct DevBuf
 struct Plan
 {
     int c, W;
-    uint32_t B, G, nb, segs;
+    uint32_t B, G, nb, segs, L;
 };
 
 // Window width for an n-point MSM: balances W*n bucket additions against the W*2^c reduction
@@ -77,14 +78,28 @@ int choose_c(size_t n)
     return c;
 }
 
+// Tuning overrides for experiments: HS_GPU_C (window bits), HS_GPU_G (buckets per reduction
+// segment, a power of two).
+int env_int(const char *name, int dflt)
+{
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
+
 Plan make_plan(size_t n)
 {
     Plan p;
-    p.c = choose_c(n);
+    p.c = env_int("HS_GPU_C", choose_c(n));
+    if (p.c < 4)
+        p.c = 4;
+    if (p.c > 16)
+        p.c = 16;
     p.W = (256 + p.c - 1) / p.c;
     p.B = 1u << (p.c - 1);
-    p.G = p.B < 16 ? p.B : 16;
+    uint32_t g = (uint32_t)env_int("HS_GPU_G", 8);
+    p.G = p.B < g ? p.B : g;
     p.nb = (uint32_t)p.W * p.B;
+    p.L = (uint32_t)env_int("HS_GPU_L", 64);
     p.segs = p.nb / p.G;
     return p;
 }
@@ -113,7 +128,7 @@ int ref_window(size_t n)
 struct Workspace
 {
     DevBuf aff, isid, digits, counts, offsets, block_sums, sorted, buckets, seg, win, big_list, big_count, entries,
-        cnt_a, off_a;
+        cnt_a, off_a, total, chunk_bucket, bnd;
 };
 
 struct Ctx
@@ -243,7 +258,12 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         (r = ws.cnt_a.reserve(((size_t)nh * nblk + 1) * 4)) || (r = ws.off_a.reserve(((size_t)nh * nblk + 1) * 4)) ||
         (r = ws.sorted.reserve(n * 4 * (size_t)P.W + 4)) || (r = ws.buckets.reserve((size_t)P.nb * sizeof(hs::xyzz))) ||
         (r = ws.seg.reserve((size_t)P.segs * sizeof(hs::xyzz))) || (r = ws.win.reserve((size_t)P.W * sizeof(hs::xyzz))) ||
-        (r = ws.big_list.reserve((size_t)P.nb * 4)) || (r = ws.big_count.reserve(64)))
+        (r = ws.big_list.reserve((size_t)P.nb * 4)) || (r = ws.big_count.reserve(64)) ||
+        (r = ws.total.reserve(sizeof(hs::jac))))
+        return r;
+    const uint32_t L = P.L;
+    const size_t max_chunks = (n * (size_t)P.W + L - 1) / L;
+    if ((r = ws.chunk_bucket.reserve(max_chunks * 4 + 4)) || (r = ws.bnd.reserve(2 * max_chunks * sizeof(hs::xyzz))))
         return r;
     uint32_t *aff = ws.aff.as<uint32_t>();
     uint8_t *isid = ws.isid.as<uint8_t>();
@@ -271,16 +291,21 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
                        offsets);
     HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
     cx.mark(st);
-    const uint32_t lmax = 96;
-    hipLaunchKernelGGL(hs::k_bucket_small<C>, dim3((P.nb + 255) / 256), blk, 0, st, offsets, P.nb, sorted, aff, lmax,
-                       buckets, big_list, big_count);
-    hipLaunchKernelGGL(hs::k_bucket_big<C>, dim3(512), dim3(hs::BIG_THREADS), 0, st, offsets, sorted, aff, big_list,
-                       big_count, buckets);
+    const dim3 gnb((P.nb + 255) / 256);
+    hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, P.nb, L, ws.chunk_bucket.as<uint32_t>());
+    hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((max_chunks + 255) / 256)), blk, 0, st, offsets, sorted,
+                       P.nb, L, ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+    hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, P.nb, L, ws.bnd.as<hs::xyzz>(), 8u, buckets,
+                       big_list, big_count);
+    hipLaunchKernelGGL(hs::k_merge_long<C>, dim3(256), dim3(hs::BIG_THREADS), 0, st, offsets, L, ws.bnd.as<hs::xyzz>(),
+                       big_list, big_count, buckets);
     cx.mark(st);
-    hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((P.segs + 255) / 256), blk, 0, st, buckets, P.B, P.G, P.segs, seg);
+    hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((P.segs + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G, P.segs,
+                       seg);
     hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(P.W), dim3(hs::BIG_THREADS), 0, st, seg, P.segs / P.W, win);
     cx.mark(st);
-    hipLaunchKernelGGL(hs::k_combine<C>, dim3(1), dim3(64), 0, st, win, P.W, P.c, d_out32, d_outjac);
+    hipLaunchKernelGGL(hs::k_horner<C>, dim3(1), dim3(64), 0, st, win, P.W, P.c, ws.total.as<hs::jac>());
+    hipLaunchKernelGGL(hs::k_finalize<C>, dim3(1), dim3(64), 0, st, ws.total.as<hs::jac>(), d_out32, d_outjac);
     cx.mark(st);
     HS_CHECK(hipGetLastError());
     return 0;

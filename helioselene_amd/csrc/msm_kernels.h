@@ -512,13 +512,153 @@ __global__ void __launch_bounds__(BIG_THREADS) k_bucket_big(const uint32_t *__re
     }
 }
 
+// ---------------------------------------------------------------- flat, balanced bucket sums
+// The sorted entries of all windows form one array grouped by global bucket id g = w*B + b.
+// Thread t sums exactly the entries [t*L, t*L + L) (every thread does the same work, whatever
+// the bucket sizes), walking across bucket boundaries:
+//   * a run that covers a whole bucket is that bucket's sum and is written to buckets[g];
+//   * a run cut by the chunk start or end is a partial sum, written to boundary slot 2t (the
+//     chunk's first run) or 2t+1 (its last run, when different from the first).
+// A bucket whose entries span chunks t0 < t1 then equals slot (2 t0 + 1, or 2 t0 when it starts
+// exactly at chunk t0's start) plus slots 2t for t0 < t <= t1; k_merge_span adds them up.
+
+// chunk_bucket[t] = the bucket holding entry t*L
+__global__ void __launch_bounds__(256) k_chunk_start(const uint32_t *__restrict__ offsets, uint32_t nb, uint32_t L,
+                                                     uint32_t *__restrict__ chunk_bucket)
+{
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nb)
+        return;
+    const uint32_t a = offsets[g], e = offsets[g + 1];
+    for (uint32_t t = (a + L - 1) / L; t * L < e; t++)
+        chunk_bucket[t] = g;
+}
+
+template <class C>
+__global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict__ offsets,
+                                                     const uint32_t *__restrict__ sorted, uint32_t nb, uint32_t L,
+                                                     const uint32_t *__restrict__ chunk_bucket,
+                                                     const uint32_t *__restrict__ pts, xyzz *__restrict__ buckets,
+                                                     xyzz *__restrict__ bnd)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t E = offsets[nb], beg = t * L;
+    if (beg >= E)
+        return;
+    const uint32_t end = min(E, beg + L);
+    uint32_t g = chunk_bucket[t];
+    uint32_t run_start = beg, bucket_end = offsets[g + 1];
+    bool first = true;
+    xyzz acc;
+    bool acc_id = true;
+    xyzz_identity<F>(acc);
+    uint32_t pos = beg;
+    uint32_t e_next = sorted[pos];
+    while (true)
+    {
+        const uint32_t e = e_next;
+        pos++;
+        if (pos < end)
+            e_next = sorted[pos]; // issued before the arithmetic on e
+        add_entry<C>(acc, acc_id, pts, e);
+        const bool run_done = pos == bucket_end || pos == end;
+        if (run_done)
+        {
+            if (acc_id)
+                xyzz_identity<F>(acc);
+            const bool whole = run_start == offsets[g] && pos == bucket_end;
+            if (whole)
+                store_xyzz(buckets + g, acc);
+            else
+                store_xyzz(bnd + 2 * (size_t)t + (first ? 0 : 1), acc);
+            if (pos == end)
+                break;
+            first = false;
+            // next non-empty bucket (entries are contiguous, so it starts at pos)
+            do
+            {
+                g++;
+                bucket_end = offsets[g + 1];
+            } while (bucket_end == pos);
+            run_start = pos;
+            acc_id = true;
+            xyzz_identity<F>(acc);
+        }
+    }
+}
+
+// Completes buckets whose entries span several chunks; spans longer than `short_max` chunks
+// are queued for k_merge_long.
+template <class C>
+__global__ void __launch_bounds__(256) k_merge_span(const uint32_t *__restrict__ offsets, uint32_t nb, uint32_t L,
+                                                    const xyzz *__restrict__ bnd, uint32_t short_max,
+                                                    xyzz *__restrict__ buckets, uint32_t *__restrict__ long_list,
+                                                    uint32_t *__restrict__ long_count)
+{
+    using F = typename C::F;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nb)
+        return;
+    const uint32_t a = offsets[g], e = offsets[g + 1];
+    if (a == e)
+        return;
+    const uint32_t t0 = a / L, t1 = (e - 1) / L;
+    if (t0 == t1)
+        return;
+    if (t1 - t0 + 1 > short_max)
+    {
+        long_list[atomicAdd(long_count, 1u)] = g;
+        return;
+    }
+    xyzz acc, p;
+    load_xyzz(acc, bnd + 2 * (size_t)t0 + (a % L ? 1 : 0));
+    for (uint32_t t = t0 + 1; t <= t1; t++)
+    {
+        load_xyzz(p, bnd + 2 * (size_t)t);
+        xyzz_add<F>(acc, acc, p);
+    }
+    store_xyzz(buckets + g, acc);
+}
+
+template <class C>
+__global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__restrict__ offsets, uint32_t L,
+                                                            const xyzz *__restrict__ bnd,
+                                                            const uint32_t *__restrict__ long_list,
+                                                            const uint32_t *__restrict__ long_count,
+                                                            xyzz *__restrict__ buckets)
+{
+    using F = typename C::F;
+    __shared__ xyzz sh[BIG_THREADS];
+    const uint32_t nlong = *long_count;
+    for (uint32_t k = blockIdx.x; k < nlong; k += gridDim.x)
+    {
+        const uint32_t g = long_list[k];
+        const uint32_t a = offsets[g], e = offsets[g + 1];
+        const uint32_t t0 = a / L, t1 = (e - 1) / L, cnt = t1 - t0 + 1;
+        xyzz acc, p;
+        xyzz_identity<F>(acc);
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x)
+        {
+            const size_t slot = j == 0 ? 2 * (size_t)t0 + (a % L ? 1 : 0) : 2 * (size_t)(t0 + j);
+            load_xyzz(p, bnd + slot);
+            xyzz_add<F>(acc, acc, p);
+        }
+        block_tree_sum<C>(acc, sh);
+        if (threadIdx.x == 0)
+            store_xyzz(buckets + g, acc);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- bucket reduction
 // Window w, segment s covers buckets [s*G, s*G + G) (bucket b holds digit magnitude b + 1).
 // Running sums give T = sum (b - s*G + 1) B_b and R = sum B_b; the segment's share of
 // sum (b+1) B_b is T + (s*G) R (reference running-sum reduction, helios_msm_vartime.cpp:334-361).
 template <class C>
-__global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buckets, uint32_t B, uint32_t G,
-                                                    uint32_t nseg_total, xyzz *__restrict__ seg_out)
+__global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buckets, const uint32_t *__restrict__ offsets,
+                                                    uint32_t B, uint32_t G, uint32_t nseg_total,
+                                                    xyzz *__restrict__ seg_out)
 {
     using F = typename C::F;
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -526,15 +666,18 @@ __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buc
         return;
     const uint32_t segs_per_win = B / G;
     const uint32_t w = t / segs_per_win, s = t % segs_per_win;
-    const xyzz *bk = buckets + (size_t)w * B + (size_t)s * G;
+    const size_t g0 = (size_t)w * B + (size_t)s * G;
     xyzz R, T;
     xyzz_identity<F>(R);
     xyzz_identity<F>(T);
     for (int j = (int)G - 1; j >= 0; j--)
     {
-        xyzz b;
-        load_xyzz(b, bk + j);
-        xyzz_add<F>(R, R, b);
+        if (offsets[g0 + j] != offsets[g0 + j + 1]) // empty buckets hold the identity
+        {
+            xyzz b;
+            load_xyzz(b, buckets + g0 + j);
+            xyzz_add<F>(R, R, b);
+        }
         xyzz_add<F>(T, T, R);
     }
     xyzz U;
@@ -565,27 +708,56 @@ __global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restri
 }
 
 // ---------------------------------------------------------------- window combine + encode
-// out32: compressed point (8 words); out_jac: reference Jacobian struct (15 uint64): (x : y : 1)
-// canonical, or (1 : 1 : 0) for the identity (helios_ops.h:44-50).
-template <class C>
-__global__ void k_combine(const xyzz *__restrict__ win_out, int W, int c, uint32_t *__restrict__ out32,
-                          uint64_t *__restrict__ out_jac)
+// Horner over the window sums in Jacobian coordinates: acc = 2^c acc + S_w, w = W-1 .. 0
+// (reference helios_msm_vartime.cpp:282-289, 366-378).  One thread; writes the Jacobian total.
+// Keeps a value in vector registers: a single-thread chain of uniform values would otherwise be
+// computed on the scalar unit, whose 32-bit multiplies are far slower than v_mad_u64_u32.
+__device__ __forceinline__ void force_vgpr(xyzz &p)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        asm volatile("" : "+v"(p.X.v[i]));
+        asm volatile("" : "+v"(p.Y.v[i]));
+        asm volatile("" : "+v"(p.ZZ.v[i]));
+        asm volatile("" : "+v"(p.ZZZ.v[i]));
+    }
+}
+
+template <class C> __global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out, int W, int c, jac *total)
 {
     using F = typename C::F;
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
-    xyzz acc;
-    load_xyzz(acc, win_out + (W - 1));
+    xyzz s;
+    jac acc, t;
+    load_xyzz(s, win_out + (W - 1));
+    force_vgpr(s);
+    xyzz_to_jac<F>(acc, s);
     for (int w = W - 2; w >= 0; w--)
     {
         for (int k = 0; k < c; k++)
-            xyzz_dbl<F>(acc, acc);
-        xyzz b;
-        load_xyzz(b, win_out + w);
-        xyzz_add<F>(acc, acc, b);
+            jac_dbl<F>(acc, acc);
+        load_xyzz(s, win_out + w);
+        force_vgpr(s);
+        xyzz_to_jac<F>(t, s);
+        jac_add<F>(acc, acc, t);
     }
+    *total = acc;
+}
+
+// out32: compressed point (8 words); out_jac: reference Jacobian struct (15 uint64): (x : y : 1)
+// canonical, or (1 : 1 : 0) for the identity (helios_ops.h:44-50).
+template <class C>
+__global__ void __launch_bounds__(64) k_finalize(const jac *__restrict__ total, uint32_t *__restrict__ out32,
+                                                 uint64_t *__restrict__ out_jac)
+{
+    using F = typename C::F;
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    jac acc = *total;
     aff a;
-    bool pt = xyzz_to_aff<F>(a, acc);
+    bool pt = jac_to_aff<F>(a, acc);
     uint32_t enc[8];
     encode_aff<F>(enc, a, pt);
     if (out32)

@@ -253,3 +253,31 @@ extern "C" int hst_recode(int curve, int c, int compat, const uint8_t *s32, uint
     }
     return W;
 }
+
+// Jacobian chain: to_jac(xyzz of a) doubled k times, plus to_jac(xyzz(b) doubled once)
+extern "C" void hst_jac_chain(int field, uint8_t *out, const uint8_t *pa, const uint8_t *pb, int k)
+{
+    g_field = field;
+    aff a, b, r;
+    ld(a.x, pa); ld(a.y, pa + 32); ld(b.x, pb); ld(b.y, pb + 32);
+    bool ok;
+    if (field == 0)
+    {
+        xyzz A, B; jac JA, JB; xyzz_from_aff<Fp>(A, a); xyzz_from_aff<Fp>(B, b);
+        xyzz_dbl<Fp>(A, A); xyzz_dbl<Fp>(B, B); // non-trivial ZZ, ZZZ
+        xyzz_to_jac<Fp>(JA, A); xyzz_to_jac<Fp>(JB, B);
+        for (int i = 0; i < k; i++) jac_dbl<Fp>(JA, JA);
+        jac_add<Fp>(JA, JA, JB);
+        ok = jac_to_aff<Fp>(r, JA);
+    }
+    else
+    {
+        xyzz A, B; jac JA, JB; xyzz_from_aff<Fq>(A, a); xyzz_from_aff<Fq>(B, b);
+        xyzz_dbl<Fq>(A, A); xyzz_dbl<Fq>(B, B);
+        xyzz_to_jac<Fq>(JA, A); xyzz_to_jac<Fq>(JB, B);
+        for (int i = 0; i < k; i++) jac_dbl<Fq>(JA, JA);
+        jac_add<Fq>(JA, JA, JB);
+        ok = jac_to_aff<Fq>(r, JA);
+    }
+    st(out, r.x); st(out + 32, r.y); out[64] = ok;
+}

@@ -165,3 +165,22 @@ def test_field_output_bounds(L, k):
             assert out.raw[39] == 0, ("limb bound", o)
             v = int.from_bytes(out.raw[:39], "little")
             assert v < (1 << 262) and v % m == want % m, (o, hex(a), hex(b))
+
+
+@pytest.mark.parametrize("c", [0, 1])
+def test_jacobian_chain_vs_oracle(L, c):
+    """xyzz -> Jacobian conversion, dbl-2001-b chains and add-2007-bl (including P + P and
+    P + (-P)) against the oracle."""
+    g = O.generator(c)
+
+    def aff(p):
+        x, y = GH.decompress(c, p)
+        return x.to_bytes(32, "little") + y.to_bytes(32, "little")
+
+    out = ctypes.create_string_buffer(65)
+    for a, b, k in ((12345, 999, 0), (12345, 999, 1), (777, 5, 13), (3, 3, 0), (3, 12, 0), (5, O.ORDER[c] - 10, 1)):
+        P = O.scalar_mul(c, a.to_bytes(32, "little"), g)
+        Q = O.scalar_mul(c, b.to_bytes(32, "little"), g)
+        L.hst_jac_chain(c, out, aff(P), aff(Q), k)
+        want = O.scalar_mul(c, ((a * 2 ** (k + 1) + 2 * b) % O.ORDER[c]).to_bytes(32, "little"), g)
+        assert _enc(out.raw) == want, (a, b, k)
