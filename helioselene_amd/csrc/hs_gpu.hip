@@ -59,7 +59,7 @@ struct DevBuf
 
 struct Plan
 {
-    int c, W;
+    int c, W, Q;
     uint32_t B, G, nb, segs, L;
 };
 
@@ -99,7 +99,10 @@ Plan make_plan(size_t n)
     uint32_t g = (uint32_t)env_int("HS_GPU_G", 8);
     p.G = p.B < g ? p.B : g;
     p.nb = (uint32_t)p.W * p.B;
-    p.L = (uint32_t)env_int("HS_GPU_L", 64);
+    p.L = (uint32_t)env_int("HS_GPU_L", 32);
+    p.Q = env_int("HS_GPU_Q", 1);
+    if (p.Q < 1)
+        p.Q = 1;
     p.segs = p.nb / p.G;
     return p;
 }
@@ -129,6 +132,21 @@ struct Workspace
 {
     DevBuf aff, isid, digits, counts, offsets, block_sums, sorted, buckets, seg, win, big_list, big_count, entries,
         cnt_a, off_a, total, chunk_bucket, bnd;
+    // second stream for the window combine, which runs behind the bucket work of lower windows
+    hipStream_t aux = nullptr;
+    static constexpr int MAXQ = 8;
+    hipEvent_t ev_group[MAXQ] = {}, ev_done = nullptr;
+
+    int ensure_aux()
+    {
+        if (aux)
+            return 0;
+        HS_CHECK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+        for (int i = 0; i < MAXQ; i++)
+            HS_CHECK(hipEventCreateWithFlags(&ev_group[i], hipEventDisableTiming));
+        HS_CHECK(hipEventCreateWithFlags(&ev_done, hipEventDisableTiming));
+        return 0;
+    }
 };
 
 struct Ctx
@@ -151,22 +169,21 @@ struct Ctx
     // phase timing (hs_gpu_profile): HIP events recorded between the phases of every plain MSM
     // issued while profiling is on, on the stream the MSM runs on
     bool profile = false;
-    static constexpr int NEV = 7; // 6 phases
+    static constexpr int NEV = 48;
     struct EvSet
     {
         hipEvent_t ev[NEV];
+        int phase[NEV]; // phase of the interval ending at ev[i]
         int curve;
         int n;
     };
     std::vector<EvSet> evsets;
     size_t evused = 0;
-    int nev = 0;
     EvSet *cur = nullptr;
 
     int begin_profile(int curve)
     {
         cur = nullptr;
-        nev = 0;
         if (!profile)
             return 0;
         if (evused == evsets.size())
@@ -182,12 +199,13 @@ struct Ctx
         return 0;
     }
 
-    void mark(hipStream_t st)
+    // records an event on st; the time since the previous mark is attributed to `phase`
+    void mark(hipStream_t st, int phase)
     {
-        if (cur && nev < NEV)
+        if (cur && cur->n < NEV)
         {
-            hipEventRecord(cur->ev[nev++], st);
-            cur->n = nev;
+            cur->phase[cur->n] = phase;
+            hipEventRecord(cur->ev[cur->n++], st);
         }
     }
 
@@ -262,7 +280,7 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         (r = ws.total.reserve(sizeof(hs::jac))))
         return r;
     const uint32_t L = P.L;
-    const size_t max_chunks = (n * (size_t)P.W + L - 1) / L;
+    const size_t max_chunks = (n * (size_t)P.W + L - 1) / L; // bound for any window group
     if ((r = ws.chunk_bucket.reserve(max_chunks * 4 + 4)) || (r = ws.bnd.reserve(2 * max_chunks * sizeof(hs::xyzz))))
         return r;
     uint32_t *aff = ws.aff.as<uint32_t>();
@@ -274,13 +292,15 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     uint32_t *big_list = ws.big_list.as<uint32_t>(), *big_count = ws.big_count.as<uint32_t>();
 
     const dim3 blk(256), grd((nn + 255) / 256);
+    if ((r = ws.ensure_aux()))
+        return r;
     if ((r = cx.begin_profile(C::id)))
         return r;
-    cx.mark(st);
+    cx.mark(st, -1);
     hipLaunchKernelGGL(hs::k_ingest<C>, grd, blk, 0, st, d_points, nn, aff, isid);
-    cx.mark(st);
+    cx.mark(st, 0);
     hipLaunchKernelGGL(hs::k_recode16<C>, grd, blk, 0, st, d_scalars, isid, nn, P.c, P.W, compat ? 1 : 0, digits);
-    cx.mark(st);
+    cx.mark(st, 1);
     hipLaunchKernelGGL(hs::k_coarse_count, dim3(nblk), blk, lds, st, digits, nn, P.W, nsb, shift, T, nblk, cnt_a);
     HS_CHECK(hipMemsetAsync(cnt_a + (size_t)nh * nblk, 0, 4, st));
     if ((r = scan_exclusive(ws, cnt_a, off_a, nh * nblk + 1, st)))
@@ -289,24 +309,43 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
                        entries);
     hipLaunchKernelGGL(hs::k_fine_sort, dim3(nh), blk, 0, st, entries, off_a, nblk, nsb, shift, P.B, nh, sorted,
                        offsets);
-    HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
-    cx.mark(st);
-    const dim3 gnb((P.nb + 255) / 256);
-    hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, P.nb, L, ws.chunk_bucket.as<uint32_t>());
-    hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((max_chunks + 255) / 256)), blk, 0, st, offsets, sorted,
-                       P.nb, L, ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
-    hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, P.nb, L, ws.bnd.as<hs::xyzz>(), 8u, buckets,
-                       big_list, big_count);
-    hipLaunchKernelGGL(hs::k_merge_long<C>, dim3(256), dim3(hs::BIG_THREADS), 0, st, offsets, L, ws.bnd.as<hs::xyzz>(),
-                       big_list, big_count, buckets);
-    cx.mark(st);
-    hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((P.segs + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G, P.segs,
-                       seg);
-    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(P.W), dim3(hs::BIG_THREADS), 0, st, seg, P.segs / P.W, win);
-    cx.mark(st);
-    hipLaunchKernelGGL(hs::k_horner<C>, dim3(1), dim3(64), 0, st, win, P.W, P.c, ws.total.as<hs::jac>());
-    hipLaunchKernelGGL(hs::k_finalize<C>, dim3(1), dim3(64), 0, st, ws.total.as<hs::jac>(), d_out32, d_outjac);
-    cx.mark(st);
+    cx.mark(st, 2);
+    // Windows are processed from the top in Q groups; group q's combine (Horner over its windows)
+    // runs on the auxiliary stream while the next group's buckets are summed on st.
+    const int Q = P.Q < Workspace::MAXQ ? P.Q : Workspace::MAXQ;
+    const uint32_t segs_per_win = P.B / P.G;
+    hs::jac *total = ws.total.as<hs::jac>();
+    for (int q = 0; q < Q; q++)
+    {
+        const int w_hi = P.W - (P.W * q) / Q, w_lo = P.W - (P.W * (q + 1)) / Q;
+        if (w_hi <= w_lo)
+            continue;
+        const uint32_t g_lo = (uint32_t)w_lo * P.B, g_hi = (uint32_t)w_hi * P.B, ng = g_hi - g_lo;
+        const dim3 gnb((ng + 255) / 256);
+        const size_t chunks = ((size_t)n * (w_hi - w_lo) + L - 1) / L;
+        HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
+        hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>());
+        hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((chunks + 255) / 256)), blk, 0, st, offsets, sorted,
+                           g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+        hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.bnd.as<hs::xyzz>(), 8u,
+                           buckets, big_list, big_count);
+        hipLaunchKernelGGL(hs::k_merge_long<C>, dim3(256), dim3(hs::BIG_THREADS), 0, st, offsets, g_lo, L,
+                           ws.bnd.as<hs::xyzz>(), big_list, big_count, buckets);
+        cx.mark(st, 3);
+        const uint32_t nseg = (uint32_t)(w_hi - w_lo) * segs_per_win;
+        hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((nseg + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G,
+                           (uint32_t)w_lo, nseg, seg);
+        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(w_hi - w_lo), dim3(hs::BIG_THREADS), 0, st, seg, segs_per_win,
+                           (uint32_t)w_lo, win);
+        cx.mark(st, 4);
+        HS_CHECK(hipEventRecord(ws.ev_group[q], st));
+        HS_CHECK(hipStreamWaitEvent(ws.aux, ws.ev_group[q], 0));
+        hipLaunchKernelGGL(hs::k_horner<C>, dim3(1), dim3(64), 0, ws.aux, win, w_lo, w_hi, P.c, q == 0 ? 1 : 0, total);
+    }
+    hipLaunchKernelGGL(hs::k_finalize<C>, dim3(1), dim3(64), 0, ws.aux, total, d_out32, d_outjac);
+    HS_CHECK(hipEventRecord(ws.ev_done, ws.aux));
+    HS_CHECK(hipStreamWaitEvent(st, ws.ev_done, 0));
+    cx.mark(st, 5);
     HS_CHECK(hipGetLastError());
     return 0;
 }
@@ -491,13 +530,16 @@ int hs_gpu_phase_summary(int curve, float *ms, int cap, int *calls)
             continue;
         HS_CHECK(hipEventSynchronize(e.ev[e.n - 1]));
         nc++;
-        for (int i = 0; i + 1 < e.n && i < cap; i++)
+        for (int i = 1; i < e.n; i++)
         {
+            const int ph = e.phase[i];
+            if (ph < 0 || ph >= cap)
+                continue;
             float t;
-            HS_CHECK(hipEventElapsedTime(&t, e.ev[i], e.ev[i + 1]));
-            ms[i] += t;
-            if (i + 1 > k)
-                k = i + 1;
+            HS_CHECK(hipEventElapsedTime(&t, e.ev[i - 1], e.ev[i]));
+            ms[ph] += t;
+            if (ph + 1 > k)
+                k = ph + 1;
         }
     }
     if (calls)
@@ -508,15 +550,26 @@ int hs_gpu_phase_summary(int curve, float *ms, int cap, int *calls)
 int hs_gpu_phase_times(float *ms, int cap)
 {
     Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    for (int i = 0; i < cap; i++)
+        ms[i] = 0.f;
+    if (cx.evused == 0)
+        return 0;
+    Ctx::EvSet &e = cx.evsets[cx.evused - 1];
+    if (e.n < 2)
+        return 0;
+    HS_CHECK(hipEventSynchronize(e.ev[e.n - 1]));
     int k = 0;
+    for (int i = 1; i < e.n; i++)
     {
-        std::lock_guard<std::mutex> lk(cx.mu);
-        if (cx.evused == 0)
-            return 0;
-        Ctx::EvSet &e = cx.evsets[cx.evused - 1];
-        HS_CHECK(hipEventSynchronize(e.ev[e.n - 1]));
-        for (int i = 0; i + 1 < e.n && i < cap; i++, k++)
-            HS_CHECK(hipEventElapsedTime(&ms[i], e.ev[i], e.ev[i + 1]));
+        const int ph = e.phase[i];
+        if (ph < 0 || ph >= cap)
+            continue;
+        float t;
+        HS_CHECK(hipEventElapsedTime(&t, e.ev[i - 1], e.ev[i]));
+        ms[ph] += t;
+        if (ph + 1 > k)
+            k = ph + 1;
     }
     return k;
 }

@@ -522,28 +522,32 @@ __global__ void __launch_bounds__(BIG_THREADS) k_bucket_big(const uint32_t *__re
 // A bucket whose entries span chunks t0 < t1 then equals slot (2 t0 + 1, or 2 t0 when it starts
 // exactly at chunk t0's start) plus slots 2t for t0 < t <= t1; k_merge_span adds them up.
 
-// chunk_bucket[t] = the bucket holding entry t*L
-__global__ void __launch_bounds__(256) k_chunk_start(const uint32_t *__restrict__ offsets, uint32_t nb, uint32_t L,
-                                                     uint32_t *__restrict__ chunk_bucket)
+// The kernels below work on the buckets [g_lo, g_hi) of a window range; chunk t covers the entries
+// base + [t*L, t*L + L), base = offsets[g_lo].
+// chunk_bucket[t] = the bucket holding entry base + t*L
+__global__ void __launch_bounds__(256) k_chunk_start(const uint32_t *__restrict__ offsets, uint32_t g_lo, uint32_t g_hi,
+                                                     uint32_t L, uint32_t *__restrict__ chunk_bucket)
 {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nb)
+    uint32_t g = g_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= g_hi)
         return;
-    const uint32_t a = offsets[g], e = offsets[g + 1];
+    const uint32_t base = offsets[g_lo];
+    const uint32_t a = offsets[g] - base, e = offsets[g + 1] - base;
     for (uint32_t t = (a + L - 1) / L; t * L < e; t++)
         chunk_bucket[t] = g;
 }
 
 template <class C>
 __global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict__ offsets,
-                                                     const uint32_t *__restrict__ sorted, uint32_t nb, uint32_t L,
+                                                     const uint32_t *__restrict__ sorted, uint32_t g_lo, uint32_t g_hi,
+                                                     uint32_t L,
                                                      const uint32_t *__restrict__ chunk_bucket,
                                                      const uint32_t *__restrict__ pts, xyzz *__restrict__ buckets,
                                                      xyzz *__restrict__ bnd)
 {
     using F = typename C::F;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t E = offsets[nb], beg = t * L;
+    const uint32_t E = offsets[g_hi], beg = offsets[g_lo] + t * L;
     if (beg >= E)
         return;
     const uint32_t end = min(E, beg + L);
@@ -591,16 +595,17 @@ __global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict_
 // Completes buckets whose entries span several chunks; spans longer than `short_max` chunks
 // are queued for k_merge_long.
 template <class C>
-__global__ void __launch_bounds__(256) k_merge_span(const uint32_t *__restrict__ offsets, uint32_t nb, uint32_t L,
-                                                    const xyzz *__restrict__ bnd, uint32_t short_max,
+__global__ void __launch_bounds__(256) k_merge_span(const uint32_t *__restrict__ offsets, uint32_t g_lo, uint32_t g_hi,
+                                                    uint32_t L, const xyzz *__restrict__ bnd, uint32_t short_max,
                                                     xyzz *__restrict__ buckets, uint32_t *__restrict__ long_list,
                                                     uint32_t *__restrict__ long_count)
 {
     using F = typename C::F;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nb)
+    const uint32_t g = g_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= g_hi)
         return;
-    const uint32_t a = offsets[g], e = offsets[g + 1];
+    const uint32_t base = offsets[g_lo];
+    const uint32_t a = offsets[g] - base, e = offsets[g + 1] - base;
     if (a == e)
         return;
     const uint32_t t0 = a / L, t1 = (e - 1) / L;
@@ -622,7 +627,7 @@ __global__ void __launch_bounds__(256) k_merge_span(const uint32_t *__restrict__
 }
 
 template <class C>
-__global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__restrict__ offsets, uint32_t L,
+__global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__restrict__ offsets, uint32_t g_lo, uint32_t L,
                                                             const xyzz *__restrict__ bnd,
                                                             const uint32_t *__restrict__ long_list,
                                                             const uint32_t *__restrict__ long_count,
@@ -634,7 +639,8 @@ __global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__re
     for (uint32_t k = blockIdx.x; k < nlong; k += gridDim.x)
     {
         const uint32_t g = long_list[k];
-        const uint32_t a = offsets[g], e = offsets[g + 1];
+        const uint32_t base = offsets[g_lo];
+        const uint32_t a = offsets[g] - base, e = offsets[g + 1] - base;
         const uint32_t t0 = a / L, t1 = (e - 1) / L, cnt = t1 - t0 + 1;
         xyzz acc, p;
         xyzz_identity<F>(acc);
@@ -657,14 +663,14 @@ __global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__re
 // sum (b+1) B_b is T + (s*G) R (reference running-sum reduction, helios_msm_vartime.cpp:334-361).
 template <class C>
 __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buckets, const uint32_t *__restrict__ offsets,
-                                                    uint32_t B, uint32_t G, uint32_t nseg_total,
+                                                    uint32_t B, uint32_t G, uint32_t w_lo, uint32_t nseg,
                                                     xyzz *__restrict__ seg_out)
 {
     using F = typename C::F;
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nseg_total)
-        return;
     const uint32_t segs_per_win = B / G;
+    uint32_t t = w_lo * segs_per_win + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w_lo * segs_per_win + nseg)
+        return;
     const uint32_t w = t / segs_per_win, s = t % segs_per_win;
     const size_t g0 = (size_t)w * B + (size_t)s * G;
     xyzz R, T;
@@ -689,11 +695,11 @@ __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buc
 // one workgroup per window: sum of the window's segment results
 template <class C>
 __global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win,
-                                                            xyzz *__restrict__ win_out)
+                                                            uint32_t w_lo, xyzz *__restrict__ win_out)
 {
     using F = typename C::F;
     __shared__ xyzz sh[BIG_THREADS];
-    const uint32_t w = blockIdx.x;
+    const uint32_t w = w_lo + blockIdx.x;
     xyzz acc;
     xyzz_identity<F>(acc);
     for (uint32_t s = threadIdx.x; s < segs_per_win; s += blockDim.x)
@@ -712,6 +718,17 @@ __global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restri
 // (reference helios_msm_vartime.cpp:282-289, 366-378).  One thread; writes the Jacobian total.
 // Keeps a value in vector registers: a single-thread chain of uniform values would otherwise be
 // computed on the scalar unit, whose 32-bit multiplies are far slower than v_mad_u64_u32.
+__device__ __forceinline__ void force_vgpr_jac(jac &p)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        asm volatile("" : "+v"(p.X.v[i]));
+        asm volatile("" : "+v"(p.Y.v[i]));
+        asm volatile("" : "+v"(p.Z.v[i]));
+    }
+}
+
 __device__ __forceinline__ void force_vgpr(xyzz &p)
 {
 #pragma unroll
@@ -724,17 +741,31 @@ __device__ __forceinline__ void force_vgpr(xyzz &p)
     }
 }
 
-template <class C> __global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out, int W, int c, jac *total)
+// Horner step over windows [w_lo, w_hi), highest first: acc = 2^c acc + S_w.  `first` starts
+// from the identity, otherwise from the Jacobian value in *total; the result goes back to *total.
+template <class C>
+__global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out, int w_lo, int w_hi, int c, int first,
+                                               jac *total)
 {
     using F = typename C::F;
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
     xyzz s;
     jac acc, t;
-    load_xyzz(s, win_out + (W - 1));
-    force_vgpr(s);
-    xyzz_to_jac<F>(acc, s);
-    for (int w = W - 2; w >= 0; w--)
+    int w = w_hi - 1;
+    if (first)
+    {
+        load_xyzz(s, win_out + w);
+        force_vgpr(s);
+        xyzz_to_jac<F>(acc, s);
+        w--;
+    }
+    else
+    {
+        acc = *total;
+        force_vgpr_jac(acc);
+    }
+    for (; w >= w_lo; w--)
     {
         for (int k = 0; k < c; k++)
             jac_dbl<F>(acc, acc);
