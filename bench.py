@@ -75,6 +75,99 @@ def make_inputs(hs, torch, curve, n, seed, dev):
     return torch.from_numpy(sc).to(dev), pts
 
 
+def make_mix(hs, torch, dev, seed, per_curve=256, lo=64, hi=1 << 14):
+    """The uneven batch: per_curve MSMs per curve, Helios and Selene interleaved, sizes
+    log-uniform in [lo, hi], distinct seeded SSWU bases per MSM, scalars uniform mod order."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    M = 2 * per_curve
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), size=M)).astype(np.int64).clip(lo, hi)
+    curves = [k % 2 for k in range(M)]
+    tot = [int(sizes[[k for k in range(M) if curves[k] == c]].sum()) for c in (0, 1)]
+    sc, pts = [], []
+    for c in (0, 1):
+        u = torch.from_numpy(rng.integers(0, 256, size=(tot[c], 32), dtype=np.uint8)).to(dev)
+        p = torch.empty((tot[c], 15), dtype=torch.int64, device=dev)
+        hs.map_to_curve_device(c, u.data_ptr(), tot[c], p.data_ptr(), None, torch.cuda.current_stream().cuda_stream)
+        s = rng.integers(0, 256, size=(tot[c], 32), dtype=np.uint8)
+        s[:, 31] &= 0x7F  # uniform 255-bit; each order is within 2^-126 of 2^255, the library reduces
+        sc.append(torch.from_numpy(s).to(dev))
+        pts.append(p)
+    torch.cuda.synchronize()
+    out = torch.zeros((M, 8), dtype=torch.int32, device=dev)
+    items, off = [], [0, 0]
+    for k in range(M):
+        c, n = curves[k], int(sizes[k])
+        items.append((c, n, sc[c].data_ptr() + 32 * off[c], pts[c].data_ptr() + 120 * off[c],
+                      out[k].data_ptr(), None))
+        off[c] += n
+    return items, out, (sc, pts), sizes
+
+
+def run_mix(hs, torch, args):
+    """Plain per-MSM launches vs one batched plan replayed from a hipGraph on the uneven mix."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        items, out, keep, sizes = make_mix(hs, torch, dev, args.seed + 1000)
+        npts = int(sizes.sum())
+        sh = st.cuda_stream
+
+        def plain_pass():
+            for (c, n, ds, dp, do, _) in items:
+                hs.msm_device(c, ds, dp, n, do, None, 0, sh)
+
+        plain_pass()
+        torch.cuda.synchronize()
+        want = out.cpu().numpy().copy()
+        t = []
+        for _ in range(args.mix_plain_steps):
+            t0 = time.perf_counter()
+            plain_pass()
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+        plain_s = float(np.median(t))
+
+        b = hs.Batch(items)
+        out.zero_()
+        b.launch(sh)
+        torch.cuda.synchronize()
+        eager_ok = bool((out.cpu().numpy() == want).all())
+        b.capture()
+        for _ in range(args.warmup):
+            b.launch(sh)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.mix_steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(st)
+            b.launch(sh)
+            e1.record(st)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+        graph_ok = bool((out.cpu().numpy() == want).all())
+        b.close()
+    batched = args.mix_steps * npts / elapsed
+    return {
+        "workload": "256 Helios + 256 Selene MSMs, sizes log-uniform in [64, 2^14], interleaved",
+        "msms": len(items),
+        "points": npts,
+        "plain_points_per_s": npts / plain_s,
+        "plain_ms_per_mix": plain_s * 1e3,
+        "batched_points_per_s": batched,
+        "batched_ms_per_mix": elapsed / args.mix_steps * 1e3,
+        "batched_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "speedup_vs_plain": batched * plain_s / npts,
+        "bit_exact_vs_plain": eager_ok and graph_ok,
+    }
+
+
 def cpu_baseline(hs_pts_cpu, sc_cpu, n_sample):
     """The reference CPU MSM (x64 Pippenger behind its runtime dispatch) on the first n_sample
     inputs of each curve, one thread."""
@@ -112,6 +205,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mix-steps", type=int, default=20, help="timed replays of the batched mix")
+    ap.add_argument("--mix-plain-steps", type=int, default=3, help="timed plain passes over the mix")
+    ap.add_argument("--no-mix", action="store_true")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams")
     args = ap.parse_args()
@@ -201,6 +297,7 @@ def main():
         "per_curve_points_per_s": {["helios", "selene"][c]: n / (sum(per_curve_ms[["helios", "selene"][c]].values()) * 1e-3)
                                    for c in (0, 1)},
     }
+    rec["batched_mix"] = None if args.no_mix else run_mix(hs, torch, args)
     if not args.no_cpu:
         pts_cpu = [inputs[c][1].cpu().numpy().view(np.uint64) for c in (0, 1)]
         sc_cpu = [inputs[c][0].cpu().numpy() for c in (0, 1)]

@@ -39,6 +39,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HsGpuError(f"HIP library not built: {LIB_PATH} (run `python -m helioselene_amd.build`)")
+        # PyTorch-ROCm ships its own HIP runtime under the same soname.  Loading torch first makes
+        # this library bind to that one copy, so device pointers and stream handles from torch
+        # are valid here and torch can still initialise the GPU after this library has.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         sz, vp, cp = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p
         L.hs_gpu_msm_ex.argtypes = [ctypes.c_int, vp, vp, vp, vp, sz, ctypes.c_uint]
@@ -58,6 +65,13 @@ def lib():
         L.hs_gpu_phase_summary.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_int)]
         L.hs_gpu_probe_int_mul_peak.argtypes = [ctypes.POINTER(ctypes.c_double)]
+        L.hs_gpu_msm_batch.argtypes = [vp, sz, ctypes.c_uint]
+        L.hs_gpu_batch_create.argtypes = [vp, sz, ctypes.c_uint, ctypes.POINTER(vp)]
+        L.hs_gpu_batch_launch.argtypes = [vp, vp]
+        L.hs_gpu_batch_capture.argtypes = [vp]
+        L.hs_gpu_batch_points.argtypes = [vp]
+        L.hs_gpu_batch_points.restype = sz
+        L.hs_gpu_batch_destroy.argtypes = [vp]
         L.hs_gpu_debug_dump.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
@@ -239,3 +253,61 @@ def phase_summary(curve=-1):
     if k < 0:
         _check(k)
     return {lib().hs_gpu_phase_name(i).decode(): float(buf[i]) for i in range(k)}, calls.value
+
+
+class MsmDesc(ctypes.Structure):
+    """hs_gpu_msm_desc of include/hs_gpu_msm.h."""
+    _fields_ = [("curve", ctypes.c_int), ("n", ctypes.c_size_t), ("scalars", ctypes.c_void_p),
+                ("points", ctypes.c_void_p), ("result", ctypes.c_void_p), ("out32", ctypes.c_void_p)]
+
+
+def msm_batch(items, flags=0):
+    """Synchronous batch over host arrays.  items: [(curve, scalars uint8 (n,32), points uint64
+    (n,15))].  Returns [(compressed 32 bytes, jacobian uint64[15])]."""
+    keep, descs, outs = [], (MsmDesc * max(1, len(items)))(), []
+    for k, (curve, sc, pts) in enumerate(items):
+        s = np.ascontiguousarray(sc, dtype=np.uint8)
+        p = np.ascontiguousarray(pts, dtype=np.uint64)
+        o = np.zeros(32, dtype=np.uint8)
+        j = np.zeros(15, dtype=np.uint64)
+        keep += [s, p]
+        outs.append((o, j))
+        descs[k] = MsmDesc(CURVES[curve], p.shape[0], s.ctypes.data, p.ctypes.data, j.ctypes.data, o.ctypes.data)
+    _check(lib().hs_gpu_msm_batch(ctypes.cast(descs, ctypes.c_void_p), len(items), flags))
+    return [(o.tobytes(), j) for o, j in outs]
+
+
+class Batch:
+    """A batch plan over DEVICE buffers (addresses as ints): create once, launch many times,
+    optionally replayed from one captured hipGraph."""
+
+    def __init__(self, items, flags=0):
+        # items: [(curve, n, d_scalars, d_points, d_out32, d_outjac or None)]
+        self._descs = (MsmDesc * max(1, len(items)))()
+        for k, (curve, n, ds, dp, do, dj) in enumerate(items):
+            self._descs[k] = MsmDesc(CURVES[curve], n, ds, dp, dj, do)
+        h = ctypes.c_void_p()
+        _check(lib().hs_gpu_batch_create(ctypes.cast(self._descs, ctypes.c_void_p), len(items), flags,
+                                         ctypes.byref(h)))
+        self.handle = h
+
+    @property
+    def points(self):
+        return lib().hs_gpu_batch_points(self.handle)
+
+    def launch(self, stream=None):
+        _check(lib().hs_gpu_batch_launch(self.handle, stream))
+
+    def capture(self):
+        _check(lib().hs_gpu_batch_capture(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().hs_gpu_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

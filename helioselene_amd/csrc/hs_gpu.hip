@@ -350,6 +350,208 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     return 0;
 }
 
+// ---------------------------------------------------------------- batched MSMs
+// Window width for MSMs of a batch (capped so one workgroup sorts a window in LDS).
+int choose_c_batch(size_t n)
+{
+    int c = choose_c(n);
+    return c > 13 ? 13 : c;
+}
+
+struct BatchGroup
+{
+    int curve = 0, c = 0, W = 0;
+    uint32_t B = 0, G = 0, M = 0, N = 0, V = 0, nb = 0, nseg = 0, L = 32;
+    size_t chunks = 0;
+    std::vector<hs::BatchDesc> hdesc;
+    std::vector<size_t> index; // position of each MSM in the caller's descriptor array
+    DevBuf desc, aff, isid, digits, sorted, offsets, buckets, seg, win, total, chunk_bucket, bnd, long_list, long_count;
+};
+
+} // namespace
+
+struct hs_gpu_batch
+{
+    std::vector<BatchGroup *> groups;
+    unsigned flags = 0;
+    std::vector<hipStream_t> workers;
+    std::vector<hipEvent_t> ev;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t total_points = 0;
+    uint32_t n_empty = 0; // MSMs with n = 0: identity outputs
+    DevBuf empty;
+};
+
+namespace
+{
+
+template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
+{
+    const dim3 blk(256);
+    const uint32_t N = g.N;
+    hs::BatchDesc *d = g.desc.as<hs::BatchDesc>();
+    uint32_t *aff = g.aff.as<uint32_t>(), *sorted = g.sorted.as<uint32_t>(), *offsets = g.offsets.as<uint32_t>();
+    hs::xyzz *buckets = g.buckets.as<hs::xyzz>(), *seg = g.seg.as<hs::xyzz>(), *win = g.win.as<hs::xyzz>();
+    hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, g.long_count.as<uint32_t>(), 1u, 0u);
+    hipLaunchKernelGGL(hs::k_ingest_batch<C>, dim3((N + 255) / 256), blk, 0, st, d, g.M, N, aff, g.isid.as<uint8_t>());
+    hipLaunchKernelGGL(hs::k_recode_batch<C>, dim3((N + 255) / 256), blk, 0, st, d, g.M, N, g.isid.as<uint8_t>(), g.c, g.W,
+                       g.digits.as<uint16_t>());
+    hipLaunchKernelGGL(hs::k_window_sort, dim3(g.V), blk, 0, st, d, N, g.W, g.B, g.digits.as<uint16_t>(), sorted, offsets,
+                       g.V);
+    const dim3 gnb((g.nb + 255) / 256);
+    hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, 0u, g.nb, g.L, g.chunk_bucket.as<uint32_t>());
+    hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((g.chunks + 255) / 256)), blk, 0, st, offsets, sorted, 0u,
+                       g.nb, g.L, g.chunk_bucket.as<uint32_t>(), aff, buckets, g.bnd.as<hs::xyzz>());
+    hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, 0u, g.nb, g.L, g.bnd.as<hs::xyzz>(), 8u, buckets,
+                       g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>());
+    hipLaunchKernelGGL(hs::k_merge_long<C>, dim3(128), dim3(hs::BIG_THREADS), 0, st, offsets, 0u, g.L,
+                       g.bnd.as<hs::xyzz>(), g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>(), buckets);
+    hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((g.nseg + 255) / 256), blk, 0, st, buckets, offsets, g.B, g.G, 0u, g.nseg,
+                       seg);
+    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(g.V), dim3(hs::BIG_THREADS), 0, st, seg, g.B / g.G, 0u, win);
+    hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
+                       g.total.as<hs::jac>());
+    hipLaunchKernelGGL(hs::k_finalize_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, g.total.as<hs::jac>(), d, g.M);
+    HS_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Enqueues every group of the batch: fork from st onto the worker streams, join back into st.
+// Works eagerly and under stream capture alike.
+int batch_enqueue(hs_gpu_batch &b, hipStream_t st)
+{
+    const size_t K = b.workers.size();
+    if (b.n_empty)
+        hipLaunchKernelGGL(hs::k_identity_out, dim3((b.n_empty + 63) / 64), dim3(64), 0, st,
+                           b.empty.as<hs::BatchDesc>(), b.n_empty);
+    HS_CHECK(hipEventRecord(b.ev[0], st));
+    for (size_t k = 0; k < K; k++)
+        HS_CHECK(hipStreamWaitEvent(b.workers[k], b.ev[0], 0));
+    for (size_t i = 0; i < b.groups.size(); i++)
+    {
+        BatchGroup &g = *b.groups[i];
+        hipStream_t ws = b.workers[i % K];
+        int r = g.curve == HS_GPU_HELIOS ? batch_group_enqueue<hs::HeliosC>(g, ws) : batch_group_enqueue<hs::SeleneC>(g, ws);
+        if (r)
+            return r;
+    }
+    for (size_t k = 0; k < K; k++)
+    {
+        HS_CHECK(hipEventRecord(b.ev[1 + k], b.workers[k]));
+        HS_CHECK(hipStreamWaitEvent(st, b.ev[1 + k], 0));
+    }
+    return 0;
+}
+
+int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
+{
+    Ctx &cx = ctx();
+    int r;
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if ((r = cx.init()))
+            return r;
+    }
+    auto *b = new hs_gpu_batch();
+    b->flags = flags;
+    // group by (curve, window width)
+    std::vector<std::pair<int, int>> keys;
+    std::vector<hs::BatchDesc> empties;
+    for (size_t i = 0; i < count; i++)
+    {
+        const hs_gpu_msm_desc &d = descs[i];
+        if ((d.curve != HS_GPU_HELIOS && d.curve != HS_GPU_SELENE) || (d.n && (!d.scalars || !d.points)))
+        {
+            delete b;
+            return fail(HS_GPU_EINVAL, "bad batch descriptor " + std::to_string(i));
+        }
+        if (d.n == 0)
+        {
+            hs::BatchDesc e{};
+            e.out32 = (uint32_t *)d.out32;
+            e.outjac = (uint64_t *)d.result;
+            empties.push_back(e);
+            continue;
+        }
+        if (d.n >= (1u << 24))
+        {
+            hs_gpu_batch_destroy(b);
+            return fail(HS_GPU_EINVAL, "batch element " + std::to_string(i) + ": n >= 2^24 (use the plain MSM)");
+        }
+        const int c = choose_c_batch(d.n);
+        BatchGroup *g = nullptr;
+        for (size_t k = 0; k < keys.size(); k++)
+            if (keys[k].first == d.curve && keys[k].second == c)
+                g = b->groups[k];
+        if (!g)
+        {
+            keys.emplace_back(d.curve, c);
+            g = new BatchGroup();
+            g->curve = d.curve;
+            g->c = c;
+            g->W = (256 + c - 1) / c;
+            g->B = 1u << (c - 1);
+            g->G = g->B < 8 ? g->B : 8;
+            b->groups.push_back(g);
+        }
+        hs::BatchDesc bd;
+        bd.scalars = (const uint32_t *)d.scalars;
+        bd.points = (const uint64_t *)d.points;
+        bd.out32 = (uint32_t *)d.out32;
+        bd.outjac = (uint64_t *)d.result;
+        bd.n = (uint32_t)d.n;
+        bd.p_off = g->N;
+        bd.w8_compat = (!(flags & HS_GPU_EXACT) && ref_window(d.n) == 8) ? 1u : 0u;
+        bd.pad = 0;
+        g->hdesc.push_back(bd);
+        g->index.push_back(i);
+        g->N += (uint32_t)d.n;
+        b->total_points += d.n;
+    }
+    for (BatchGroup *g : b->groups)
+    {
+        g->M = (uint32_t)g->hdesc.size();
+        g->V = g->M * (uint32_t)g->W;
+        g->nb = g->V * g->B;
+        g->nseg = g->nb / g->G;
+        g->chunks = ((size_t)g->N * g->W + g->L - 1) / g->L;
+        const size_t E = (size_t)g->N * g->W;
+        if ((r = g->desc.reserve(g->M * sizeof(hs::BatchDesc))) || (r = g->aff.reserve((size_t)g->N * 64)) ||
+            (r = g->isid.reserve(g->N)) || (r = g->digits.reserve(E * 2)) || (r = g->sorted.reserve(E * 4 + 4)) ||
+            (r = g->offsets.reserve(((size_t)g->nb + 1) * 4)) || (r = g->buckets.reserve((size_t)g->nb * sizeof(hs::xyzz))) ||
+            (r = g->seg.reserve((size_t)g->nseg * sizeof(hs::xyzz))) || (r = g->win.reserve((size_t)g->V * sizeof(hs::xyzz))) ||
+            (r = g->total.reserve((size_t)g->M * sizeof(hs::jac))) || (r = g->chunk_bucket.reserve(g->chunks * 4 + 4)) ||
+            (r = g->bnd.reserve(2 * g->chunks * sizeof(hs::xyzz))) || (r = g->long_list.reserve((size_t)g->nb * 4)) ||
+            (r = g->long_count.reserve(64)))
+        {
+            hs_gpu_batch_destroy(b);
+            return r;
+        }
+        HS_CHECK(hipMemcpy(g->desc.p, g->hdesc.data(), g->M * sizeof(hs::BatchDesc), hipMemcpyHostToDevice));
+    }
+    if (!empties.empty())
+    {
+        b->n_empty = (uint32_t)empties.size();
+        if ((r = b->empty.reserve(empties.size() * sizeof(hs::BatchDesc))))
+        {
+            hs_gpu_batch_destroy(b);
+            return r;
+        }
+        HS_CHECK(hipMemcpy(b->empty.p, empties.data(), empties.size() * sizeof(hs::BatchDesc), hipMemcpyHostToDevice));
+    }
+    const size_t K = b->groups.size() < 4 ? (b->groups.empty() ? 1 : b->groups.size()) : 4;
+    b->workers.resize(K);
+    b->ev.resize(K + 1);
+    for (size_t k = 0; k < K; k++)
+        HS_CHECK(hipStreamCreateWithFlags(&b->workers[k], hipStreamNonBlocking));
+    for (size_t k = 0; k < K + 1; k++)
+        HS_CHECK(hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming));
+    *out = b;
+    return 0;
+}
+
 const char *const PHASE_NAMES[] = {"ingest", "recode", "sort", "bucket", "reduce", "combine"};
 
 template <class C>
@@ -502,6 +704,152 @@ int hs_gpu_init(void)
 }
 
 const char *hs_gpu_last_error(void) { return g_err.c_str(); }
+
+int hs_gpu_batch_create(const hs_gpu_msm_desc *descs, size_t count, unsigned flags, hs_gpu_batch **out)
+{
+    if (!out || (count && !descs))
+        return fail(HS_GPU_EINVAL, "null argument");
+    return batch_create(out, descs, count, flags);
+}
+
+int hs_gpu_batch_launch(hs_gpu_batch *b, void *stream)
+{
+    if (!b)
+        return fail(HS_GPU_EINVAL, "null batch");
+    hipStream_t st = stream ? (hipStream_t)stream : ctx().stream;
+    if (b->exec)
+    {
+        HS_CHECK(hipGraphLaunch(b->exec, st));
+        return 0;
+    }
+    return batch_enqueue(*b, st);
+}
+
+int hs_gpu_batch_capture(hs_gpu_batch *b)
+{
+    if (!b)
+        return fail(HS_GPU_EINVAL, "null batch");
+    if (b->exec)
+        return 0;
+    if (!b->cap_stream)
+        HS_CHECK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    HS_CHECK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    int r = batch_enqueue(*b, b->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (r)
+        return r;
+    if (e != hipSuccess)
+        return fail(HS_GPU_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    HS_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+size_t hs_gpu_batch_points(const hs_gpu_batch *b) { return b ? b->total_points : 0; }
+
+void hs_gpu_batch_destroy(hs_gpu_batch *b)
+{
+    if (!b)
+        return;
+    if (b->exec)
+        hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        hipGraphDestroy(b->graph);
+    for (BatchGroup *g : b->groups)
+    {
+        for (DevBuf *d : {&g->desc, &g->aff, &g->isid, &g->digits, &g->sorted, &g->offsets, &g->buckets, &g->seg, &g->win,
+                          &g->total, &g->chunk_bucket, &g->bnd, &g->long_list, &g->long_count})
+            if (d->p)
+                hipFree(d->p);
+        delete g;
+    }
+    for (hipStream_t s : b->workers)
+        hipStreamDestroy(s);
+    for (hipEvent_t e : b->ev)
+        hipEventDestroy(e);
+    if (b->cap_stream)
+        hipStreamDestroy(b->cap_stream);
+    if (b->empty.p)
+        hipFree(b->empty.p);
+    delete b;
+}
+
+int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
+{
+    if (count && !descs)
+        return fail(HS_GPU_EINVAL, "null descriptors");
+    // stage every input in one device buffer, run the batch, copy the outputs back
+    size_t n_tot = 0;
+    for (size_t i = 0; i < count; i++)
+        n_tot += descs[i].n;
+    Ctx &cx = ctx();
+    int r;
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if ((r = cx.init()))
+            return r;
+    }
+    DevBuf in, outb;
+    const size_t in_bytes = n_tot * (32 + sizeof(hs_gpu_jacobian));
+    if ((r = in.reserve(in_bytes + 64)) || (r = outb.reserve(count * (32 + sizeof(hs_gpu_jacobian)) + 64)))
+        return r;
+    std::vector<hs_gpu_msm_desc> dd(descs, descs + count);
+    unsigned char *sc_base = in.as<unsigned char>(), *pt_base = sc_base + n_tot * 32;
+    unsigned char *o32_base = outb.as<unsigned char>(), *oj_base = o32_base + count * 32;
+    hipStream_t st = cx.stream;
+    size_t off = 0;
+    for (size_t i = 0; i < count; i++)
+    {
+        const size_t n = descs[i].n;
+        if (n)
+        {
+            HS_CHECK(hipMemcpyAsync(sc_base + off * 32, descs[i].scalars, n * 32, hipMemcpyHostToDevice, st));
+            HS_CHECK(hipMemcpyAsync(pt_base + off * sizeof(hs_gpu_jacobian), descs[i].points, n * sizeof(hs_gpu_jacobian),
+                                    hipMemcpyHostToDevice, st));
+        }
+        dd[i].scalars = sc_base + off * 32;
+        dd[i].points = (const hs_gpu_jacobian *)(pt_base + off * sizeof(hs_gpu_jacobian));
+        dd[i].out32 = o32_base + i * 32;
+        dd[i].result = (hs_gpu_jacobian *)(oj_base + i * sizeof(hs_gpu_jacobian));
+        off += n;
+    }
+    HS_CHECK(hipStreamSynchronize(st));
+    hs_gpu_batch *b = nullptr;
+    if ((r = batch_create(&b, dd.data(), count, flags)))
+        return r;
+    r = batch_enqueue(*b, st);
+    if (!r)
+    {
+        std::vector<unsigned char> o32(count * 32);
+        std::vector<hs_gpu_jacobian> oj(count);
+        hipError_t e1 = hipMemcpyAsync(o32.data(), o32_base, count * 32, hipMemcpyDeviceToHost, st);
+        hipError_t e2 = hipMemcpyAsync(oj.data(), oj_base, count * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToHost, st);
+        hipError_t e3 = hipStreamSynchronize(st);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+            r = fail(HS_GPU_EDEVICE, "batch result copy failed");
+        for (size_t i = 0; i < count && !r; i++)
+        {
+            if (descs[i].n == 0)
+            {
+                if (descs[i].out32)
+                    std::memset(descs[i].out32, 0, 32);
+                if (descs[i].result)
+                    set_identity(descs[i].result);
+                continue;
+            }
+            if (descs[i].out32)
+                std::memcpy(descs[i].out32, &o32[i * 32], 32);
+            if (descs[i].result)
+                descs[i].result[0] = oj[i];
+        }
+    }
+    hs_gpu_batch_destroy(b);
+    hipFree(in.p);
+    hipFree(outb.p);
+    in.p = outb.p = nullptr;
+    return r;
+}
 
 int hs_gpu_profile(int on)
 {

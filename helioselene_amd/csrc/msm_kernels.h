@@ -418,10 +418,14 @@ __global__ void __launch_bounds__(256) k_fine_sort(const uint32_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------- bucket sums
+constexpr uint32_t ENTRY_SKIP = 0xffffffffu; // padding entry of the batched layout
+
 template <class C>
 __device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const uint32_t *__restrict__ pts, uint32_t e)
 {
     using F = typename C::F;
+    if (e == ENTRY_SKIP)
+        return;
     aff p;
     load_aff(p, pts, e & 0x7fffffffu);
     if (e >> 31)
@@ -945,5 +949,259 @@ __global__ void __launch_bounds__(64) k_encode_xyzz(const xyzz *__restrict__ p, 
     encode_aff<F>(e, a, pt);
     for (int k = 0; k < 8; k++)
         out32[8ull * i + k] = e[k];
+}
+} // namespace hs
+
+
+// ================================================================ batched MSMs
+// A group of M MSMs of one curve sharing the window width c.  Points of all MSMs are concatenated
+// (MSM m owns global points [p_off[m], p_off[m] + n[m])); each MSM has W windows, and virtual
+// window v = m*W + w owns the entry slots [p_off[m]*W + w*n[m], ... + n[m]) of the sorted array.
+// Zero digits become ENTRY_SKIP slots at the end of the window's last bucket, so the layout, every
+// buffer size and every grid depend only on the batch shape.
+namespace hs
+{
+
+struct BatchDesc
+{
+    const uint32_t *scalars; // n x 8 words (device)
+    const uint64_t *points;  // n x 15 uint64 (reference Jacobian struct, device)
+    uint32_t *out32;         // 8 words (device), may be null
+    uint64_t *outjac;        // 15 uint64 (device), may be null
+    uint32_t n, p_off, w8_compat, pad;
+};
+
+__device__ __forceinline__ uint32_t find_msm(const BatchDesc *__restrict__ d, uint32_t M, uint32_t i)
+{
+    uint32_t lo = 0, hi = M; // largest m with p_off[m] <= i
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (d[mid].p_off <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <class C>
+__global__ void __launch_bounds__(256) k_ingest_batch(const BatchDesc *__restrict__ d, uint32_t M, uint32_t N,
+                                                      uint32_t *__restrict__ out, uint8_t *__restrict__ is_id)
+{
+    using F = typename C::F;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const uint32_t m = find_msm(d, M, i), j = i - d[m].p_off;
+    const uint64_t *p = d[m].points + 15ull * j;
+    uint64_t l[5];
+    fe X, Y, Z;
+    for (int k = 0; k < 5; k++)
+        l[k] = p[k];
+    fe_from_r51<F>(X, l);
+    for (int k = 0; k < 5; k++)
+        l[k] = p[5 + k];
+    fe_from_r51<F>(Y, l);
+    for (int k = 0; k < 5; k++)
+        l[k] = p[10 + k];
+    fe_from_r51<F>(Z, l);
+    aff a;
+    uint32_t Zc[8];
+    fe_to_words_canon<F>(Zc, Z);
+    uint32_t zor = 0, one = Zc[0] == 1u;
+    for (int k = 0; k < 8; k++)
+        zor |= Zc[k];
+    for (int k = 1; k < 8; k++)
+        one &= Zc[k] == 0u;
+    if (zor == 0)
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+        is_id[i] = 1;
+    }
+    else
+    {
+        if (one)
+        {
+            a.x = X;
+            a.y = Y;
+        }
+        else
+        {
+            fe zi, zi2, zi3;
+            fe_inv<F>(zi, Z);
+            fe_sq<F>(zi2, zi);
+            fe_mul<F>(zi3, zi2, zi);
+            fe_mul<F>(a.x, X, zi2);
+            fe_mul<F>(a.y, Y, zi3);
+        }
+        is_id[i] = 0;
+    }
+    store_aff<F>(out, i, a);
+}
+
+template <class C>
+__global__ void __launch_bounds__(256) k_recode_batch(const BatchDesc *__restrict__ d, uint32_t M, uint32_t N,
+                                                      const uint8_t *__restrict__ is_id, int c, int W,
+                                                      uint16_t *__restrict__ digits)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const uint32_t m = find_msm(d, M, i), j = i - d[m].p_off;
+    const uint4 *sp = reinterpret_cast<const uint4 *>(d[m].scalars + 8ull * j);
+    uint4 a = sp[0], b = sp[1];
+    const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t k[8];
+    scalar_prepare<typename C::O>(k, s, d[m].w8_compat != 0);
+    const bool skip = is_id[i] != 0;
+    uint32_t carry = 0;
+    for (int w = 0; w < W; w++)
+    {
+        uint32_t dg = recode_digit(k, c, w, carry);
+        digits[(size_t)w * N + i] =
+            (dg == 0 || skip) ? DIGIT_ZERO : (uint16_t)(((dg & 0x7fffffffu) - 1) | ((dg >> 31) << 15));
+    }
+}
+
+// One block per virtual window: LDS counting sort of the window's n digits into B buckets.
+constexpr int WSORT_MAX_B = 4096;
+
+__global__ void __launch_bounds__(256) k_window_sort(const BatchDesc *__restrict__ d, uint32_t N, int W, uint32_t B,
+                                                     const uint16_t *__restrict__ digits, uint32_t *__restrict__ sorted,
+                                                     uint32_t *__restrict__ offsets, uint32_t V)
+{
+    __shared__ uint32_t cnt[WSORT_MAX_B];
+    __shared__ uint32_t part[256];
+    const uint32_t v = blockIdx.x, m = v / W, w = v % W;
+    const uint32_t n = d[m].n, p_off = d[m].p_off;
+    const uint32_t base = p_off * (uint32_t)W + w * n;
+    const uint16_t *dg = digits + (size_t)w * N + p_off;
+    for (uint32_t b = threadIdx.x; b < B; b += blockDim.x)
+        cnt[b] = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x)
+    {
+        const uint16_t code = dg[j];
+        if (code != DIGIT_ZERO)
+            atomicAdd(&cnt[code & 0x7fffu], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of B counters: each thread scans a contiguous slice, then the slice totals
+    const uint32_t per = (B + blockDim.x - 1) / blockDim.x, b0 = threadIdx.x * per;
+    uint32_t tot = 0;
+    for (uint32_t b = b0; b < b0 + per && b < B; b++)
+        tot += cnt[b];
+    part[threadIdx.x] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < blockDim.x; t++)
+        {
+            const uint32_t x = part[t];
+            part[t] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    uint32_t run = base + part[threadIdx.x];
+    for (uint32_t b = b0; b < b0 + per && b < B; b++)
+    {
+        const uint32_t x = cnt[b];
+        cnt[b] = run; // becomes the scatter cursor
+        offsets[(size_t)v * B + b] = run;
+        run += x;
+    }
+    if (v == V - 1 && threadIdx.x == 0)
+        offsets[(size_t)V * B] = base + n;
+    __syncthreads();
+    // nonzero digits into their buckets; the remaining slots of the window are padding
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x)
+    {
+        const uint16_t code = dg[j];
+        if (code != DIGIT_ZERO)
+        {
+            const uint32_t pos = atomicAdd(&cnt[code & 0x7fffu], 1u);
+            sorted[pos] = (p_off + j) | ((uint32_t)(code >> 15) << 31);
+        }
+    }
+    __syncthreads();
+    // after the scatter cnt[B-1] points one past the last real entry of the window
+    for (uint32_t j = cnt[B - 1] + threadIdx.x; j < base + n; j += blockDim.x)
+        sorted[j] = ENTRY_SKIP;
+}
+
+// Horner per MSM (one thread each) over its W window sums, then the affine encoding.
+template <class C>
+__global__ void __launch_bounds__(64) k_horner_batch(const xyzz *__restrict__ win_out, uint32_t M, int W, int c,
+                                                     jac *__restrict__ total)
+{
+    using F = typename C::F;
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M)
+        return;
+    const xyzz *ws = win_out + (size_t)m * W;
+    xyzz s;
+    jac acc, t;
+    load_xyzz(s, ws + (W - 1));
+    xyzz_to_jac<F>(acc, s);
+    for (int w = W - 2; w >= 0; w--)
+    {
+        for (int k = 0; k < c; k++)
+            jac_dbl<F>(acc, acc);
+        load_xyzz(s, ws + w);
+        xyzz_to_jac<F>(t, s);
+        jac_add<F>(acc, acc, t);
+    }
+    total[m] = acc;
+}
+
+template <class C>
+__global__ void __launch_bounds__(64) k_finalize_batch(const jac *__restrict__ total, const BatchDesc *__restrict__ d,
+                                                       uint32_t M)
+{
+    using F = typename C::F;
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M)
+        return;
+    jac acc = total[m];
+    aff a;
+    const bool pt = jac_to_aff<F>(a, acc);
+    uint32_t enc[8];
+    encode_aff<F>(enc, a, pt);
+    if (d[m].out32)
+        for (int k = 0; k < 8; k++)
+            d[m].out32[k] = enc[k];
+    if (d[m].outjac)
+        store_r51_aff<F>(d[m].outjac, a, pt);
+}
+
+} // namespace hs
+
+namespace hs
+{
+// identity outputs for the empty MSMs (n = 0) of a batch: 0^32 and the reference's identity
+// Jacobian (X = 1, Y = 1, Z = 0 in radix 2^51)
+__global__ void k_identity_out(const BatchDesc *__restrict__ d, uint32_t M)
+{
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M)
+        return;
+    if (d[m].out32)
+        for (int k = 0; k < 8; k++)
+            d[m].out32[k] = 0;
+    if (d[m].outjac)
+        for (int k = 0; k < 15; k++)
+            d[m].outjac[k] = (k == 0 || k == 5) ? 1u : 0u;
+}
+
+// graph-safe fill (captured graphs get no memset nodes)
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
 }
 } // namespace hs

@@ -90,6 +90,35 @@ int hs_gpu_synchronize(void);
 int hs_gpu_msm_bytes(int curve, unsigned char out32[32], const unsigned char *scalars, const unsigned char *points32,
                      size_t n, unsigned flags);
 
+/* ---------------------------------------------------------------- batched MSMs
+ * Many independent MSMs of uneven sizes, both curves mixed, computed by one launch sequence:
+ * MSMs are grouped by curve and window width, every group's work runs as one set of kernels over
+ * all of its MSMs, and the groups run on parallel streams.  The reference has no batch call: each
+ * element is one helios_msm_vartime / selene_msm_vartime (same formats and result semantics).
+ * A descriptor's scalars/points are n x 32 bytes / n reference Jacobian structs; result and out32
+ * may be NULL.  hs_gpu_msm_batch takes host pointers and is synchronous.  The plan form takes
+ * DEVICE pointers: create once (allocates and uploads the layout), launch many times
+ * asynchronously; hs_gpu_batch_capture records the whole launch sequence into one hipGraph,
+ * after which hs_gpu_batch_launch replays that graph. */
+typedef struct hs_gpu_msm_desc
+{
+    int curve;
+    size_t n;
+    const unsigned char *scalars;
+    const hs_gpu_jacobian *points;
+    hs_gpu_jacobian *result;
+    unsigned char *out32;
+} hs_gpu_msm_desc;
+
+typedef struct hs_gpu_batch hs_gpu_batch;
+
+int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags);
+int hs_gpu_batch_create(const hs_gpu_msm_desc *device_descs, size_t count, unsigned flags, hs_gpu_batch **out);
+int hs_gpu_batch_launch(hs_gpu_batch *batch, void *stream);
+int hs_gpu_batch_capture(hs_gpu_batch *batch);
+size_t hs_gpu_batch_points(const hs_gpu_batch *batch);
+void hs_gpu_batch_destroy(hs_gpu_batch *batch);
+
 /* ---------------------------------------------------------------- hash-to-curve
  * Simplified SWU map of n 32-byte inputs (RFC 9380 6.6.2; reference helios_map_to_curve /
  * selene_map_to_curve, helios/src/x64/helios_map_to_curve.cpp:184-189).  Outputs the reference
