@@ -72,6 +72,19 @@ def lib():
         L.hs_gpu_batch_points.argtypes = [vp]
         L.hs_gpu_batch_points.restype = sz
         L.hs_gpu_batch_destroy.argtypes = [vp]
+        L.hs_gpu_helios_pedersen_commit.argtypes = [vp, vp, vp, vp, vp, sz]
+        L.hs_gpu_selene_pedersen_commit.argtypes = [vp, vp, vp, vp, vp, sz]
+        L.hs_gpu_helios_msm_fixed.argtypes = [vp, vp, vp, sz]
+        L.hs_gpu_selene_msm_fixed.argtypes = [vp, vp, vp, sz]
+        L.hs_gpu_fixed_precompute.argtypes = [ctypes.c_int, vp, sz, vp]
+        L.hs_gpu_fixed_create.argtypes = [ctypes.c_int, vp, sz, ctypes.POINTER(vp)]
+        L.hs_gpu_fixed_create_device.argtypes = [ctypes.c_int, vp, sz, vp, ctypes.POINTER(vp)]
+        L.hs_gpu_fixed_size.argtypes = [vp]
+        L.hs_gpu_fixed_size.restype = sz
+        L.hs_gpu_fixed_msm.argtypes = [vp, vp, sz, ctypes.c_uint, ctypes.c_int, vp, vp]
+        L.hs_gpu_fixed_msm_device.argtypes = [vp, vp, sz, ctypes.c_uint, ctypes.c_int, vp, vp, vp]
+        L.hs_gpu_pedersen_commit_batch.argtypes = [vp, vp, vp, sz, ctypes.c_uint, ctypes.c_int, vp, vp]
+        L.hs_gpu_fixed_destroy.argtypes = [vp]
         L.hs_gpu_debug_dump.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
@@ -144,6 +157,8 @@ def scalars_array(scalars):
         buf += s if isinstance(s, (bytes, bytearray)) else int(s).to_bytes(32, "little")
     return np.frombuffer(bytes(buf), dtype=np.uint8).reshape(-1, 32).copy()
 
+
+FIXED_AUTO, FIXED_COMB, FIXED_PIPPENGER = 0, 1, 2
 
 # ---------------------------------------------------------------- MSM calls
 def msm(curve, scalars, points, flags=0):
@@ -304,6 +319,92 @@ class Batch:
     def close(self):
         if self.handle:
             lib().hs_gpu_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- Pedersen / fixed-base
+def pedersen_commit(curve, blinding, H, values, generators):
+    """The void drop-in (hs_gpu_helios_pedersen_commit / selene): blinding 32 bytes, H uint64[15],
+    values uint8 (n, 32), generators uint64 (n, 15).  Returns the result struct uint64[15]."""
+    c = CURVES[curve]
+    b = np.frombuffer(bytes(blinding), dtype=np.uint8).copy()
+    h = np.ascontiguousarray(H, dtype=np.uint64).reshape(15)
+    v = np.ascontiguousarray(values, dtype=np.uint8).reshape(-1, 32)
+    g = np.ascontiguousarray(generators, dtype=np.uint64).reshape(-1, 15)
+    res = np.zeros(15, dtype=np.uint64)
+    f = lib().hs_gpu_helios_pedersen_commit if c == HELIOS else lib().hs_gpu_selene_pedersen_commit
+    f(_ptr(res), _ptr(b), _ptr(h), _ptr(v), _ptr(g), g.shape[0])
+    return res
+
+
+def fixed_precompute(curve, points):
+    """Width-5 tables of n bases: uint64 (n, 16, 10), entry k of base i = (k+1) P_i as the
+    reference's affine struct (x[5], y[5] radix 2^51)."""
+    p = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1, 15)
+    t = np.zeros((p.shape[0], 16, 10), dtype=np.uint64)
+    _check(lib().hs_gpu_fixed_precompute(CURVES[curve], _ptr(p), p.shape[0], _ptr(t)))
+    return t
+
+
+def msm_fixed(curve, scalars, tables):
+    """The void drop-in (hs_gpu_helios_msm_fixed / selene) over per-base tables uint64 (n, 16, 10);
+    passes an array of n table pointers like the reference's `const helios_affine *const *`."""
+    c = CURVES[curve]
+    s = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1, 32)
+    t = np.ascontiguousarray(tables, dtype=np.uint64).reshape(-1, 16, 10)
+    n = t.shape[0]
+    ptrs = (ctypes.c_void_p * max(1, n))(*[t.ctypes.data + i * 16 * 80 for i in range(n)])
+    res = np.zeros(15, dtype=np.uint64)
+    f = lib().hs_gpu_helios_msm_fixed if c == HELIOS else lib().hs_gpu_selene_msm_fixed
+    f(_ptr(res), _ptr(s), ctypes.cast(ptrs, ctypes.c_void_p), n)
+    return res
+
+
+class FixedSet:
+    """A resident fixed-base set (bases and width-5 tables kept in HBM)."""
+
+    def __init__(self, curve, points=None, d_points=None, n=None, stream=None):
+        self.curve = CURVES[curve]
+        h = ctypes.c_void_p()
+        if points is not None:
+            p = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1, 15)
+            _check(lib().hs_gpu_fixed_create(self.curve, _ptr(p), p.shape[0], ctypes.byref(h)))
+        else:
+            _check(lib().hs_gpu_fixed_create_device(self.curve, d_points, n, stream, ctypes.byref(h)))
+        self.handle = h
+        self.n = lib().hs_gpu_fixed_size(h)
+
+    def msm(self, scalars, flags=0, method=FIXED_AUTO):
+        """scalars uint8 (count, n, 32) -> (list of 32-byte results, uint64 (count, 15))."""
+        s = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1, self.n, 32)
+        k = s.shape[0]
+        o = np.zeros((k, 32), dtype=np.uint8)
+        j = np.zeros((k, 15), dtype=np.uint64)
+        _check(lib().hs_gpu_fixed_msm(self.handle, _ptr(s), k, flags, method, _ptr(o), _ptr(j)))
+        return [o[i].tobytes() for i in range(k)], j
+
+    def pedersen(self, blindings, values, flags=0, method=FIXED_AUTO):
+        """Set = [H, G_0..G_{n-2}]: blindings (count, 32), values (count, n-1, 32)."""
+        b = np.ascontiguousarray(blindings, dtype=np.uint8).reshape(-1, 32)
+        k = b.shape[0]
+        v = np.ascontiguousarray(values, dtype=np.uint8).reshape(k, self.n - 1, 32)
+        o = np.zeros((k, 32), dtype=np.uint8)
+        j = np.zeros((k, 15), dtype=np.uint64)
+        _check(lib().hs_gpu_pedersen_commit_batch(self.handle, _ptr(b), _ptr(v), k, flags, method, _ptr(o), _ptr(j)))
+        return [o[i].tobytes() for i in range(k)], j
+
+    def msm_device(self, d_scalars, count, d_out32, d_out_jac=None, flags=0, method=FIXED_AUTO, stream=None):
+        _check(lib().hs_gpu_fixed_msm_device(self.handle, d_scalars, count, flags, method, d_out32, d_out_jac, stream))
+
+    def close(self):
+        if self.handle:
+            lib().hs_gpu_fixed_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

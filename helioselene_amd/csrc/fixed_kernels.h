@@ -1,0 +1,226 @@
+// Fixed-base multi-scalar multiplication over resident width-5 tables.
+//
+// Restates the algorithm of helios_msm_fixed (helios/include/helios_msm_fixed.h:60-155) and its
+// tables (helios_scalarmult_fixed_precompute, helios/include/helios_scalarmult_fixed.h:54-65):
+// every base P_i carries the 16 affine multiples [1P_i .. 16P_i]; scalars are split into 52
+// signed 5-bit digits, and the result is
+//     sum_w 32^w * S_w,   S_w = sum_i sign(d_iw) * T_i[|d_iw|]
+// so all MSMs over one base set share the 5 doublings per window (255 in total).
+//
+// GPU mapping: one wavefront per (MSM k, window w) accumulates S_kw over all bases (lane l takes
+// bases l, l+64, ...) with mixed additions, then a 64-way LDS tree sum.  One thread per MSM runs
+// the Horner chain over its 52 window sums.  The digit of window w is read locally from the
+// canonical scalar: d = bits[5w, 5w+5) + bit(5w-1) - 32 * bit(5w+4), in [-16, 16]; this signed
+// recoding differs from helios_scalar_recode_signed5 digit by digit but represents the same
+// integer, and group arithmetic here is exact, so every result is the same point.
+//
+// Table entries are canonical affine (x, y) as 16 little-endian words; an all-zero entry is the
+// identity (the table of an identity base).
+#pragma once
+#include "msm_kernels.h"
+
+namespace hs
+{
+
+constexpr int FIXED_WINDOWS = 52;
+constexpr int FIXED_ENTRIES = 16;
+
+// Tables from reference Jacobian structs: thread (i, k) computes (k+1) * P_i.  Optionally also
+// writes the reference's helios_affine layout (x[5], y[5] radix-2^51, canonical).
+template <class C>
+__global__ void __launch_bounds__(256) k_fixed_tables(const uint64_t *__restrict__ pts, uint32_t n,
+                                                      uint32_t *__restrict__ tables, uint64_t *__restrict__ ref_out)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * FIXED_ENTRIES)
+        return;
+    const uint32_t i = t / FIXED_ENTRIES, k = t % FIXED_ENTRIES;
+    const uint64_t *p = pts + 15ull * i;
+    uint64_t l[5];
+    fe X, Y, Z;
+    for (int j = 0; j < 5; j++)
+        l[j] = p[j];
+    fe_from_r51<F>(X, l);
+    for (int j = 0; j < 5; j++)
+        l[j] = p[5 + j];
+    fe_from_r51<F>(Y, l);
+    for (int j = 0; j < 5; j++)
+        l[j] = p[10 + j];
+    fe_from_r51<F>(Z, l);
+    aff a;
+    bool pt = !fe_is_zero<F>(Z);
+    if (pt)
+    {
+        jac J;
+        J.X = X;
+        J.Y = Y;
+        J.Z = Z;
+        aff base;
+        jac_to_aff<F>(base, J);
+        xyzz b, m;
+        xyzz_from_aff<F>(b, base);
+        xyzz_mul_small<F>(m, b, k + 1);
+        pt = xyzz_to_aff<F>(a, m);
+    }
+    if (!pt)
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+    }
+    store_aff<F>(tables, t, a);
+    if (ref_out)
+    {
+        uint64_t *o = ref_out + 10ull * t;
+        fe_to_r51<F>(o, a.x);
+        fe_to_r51<F>(o + 5, a.y);
+    }
+}
+
+// Tables given in the reference's helios_affine layout (possibly lazy limbs): canonicalise.
+// (0, 0) stays the all-zero identity entry.
+template <class C>
+__global__ void __launch_bounds__(256) k_fixed_tables_from_ref(const uint64_t *__restrict__ ref_in, uint32_t entries,
+                                                               uint32_t *__restrict__ tables)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= entries)
+        return;
+    const uint64_t *p = ref_in + 10ull * t;
+    uint64_t l[5];
+    aff a;
+    fe v;
+    for (int j = 0; j < 5; j++)
+        l[j] = p[j];
+    fe_from_r51<F>(v, l);
+    fe_canon<F>(a.x, v);
+    for (int j = 0; j < 5; j++)
+        l[j] = p[5 + j];
+    fe_from_r51<F>(v, l);
+    fe_canon<F>(a.y, v);
+    store_aff<F>(tables, t, a);
+}
+
+// Scalars -> canonical words mod the group order, with the reference's window-8 substitution
+// when `w8_compat` is set (each row then matches helios_msm_vartime over the same bases).
+template <class C>
+__global__ void __launch_bounds__(256) k_fixed_prep(const uint32_t *__restrict__ in, size_t count,
+                                                    uint32_t *__restrict__ out, uint32_t w8_compat)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count)
+        return;
+    const uint4 *sp = reinterpret_cast<const uint4 *>(in + 8 * t);
+    const uint4 a = sp[0], b = sp[1];
+    const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t k[8];
+    scalar_prepare<typename C::O>(k, s, w8_compat != 0);
+    uint4 *op = reinterpret_cast<uint4 *>(out + 8 * t);
+    op[0] = make_uint4(k[0], k[1], k[2], k[3]);
+    op[1] = make_uint4(k[4], k[5], k[6], k[7]);
+}
+
+// signed 5-bit digit of window w of a canonical scalar (< 2^255)
+__device__ __forceinline__ int fixed_digit(const uint32_t *__restrict__ s, int w)
+{
+    uint32_t v;
+    if (w == 0)
+        v = (s[0] << 1) & 63u;
+    else
+    {
+        const int bit = 5 * w - 1, wi = bit >> 5, off = bit & 31;
+        const uint64_t lo = s[wi];
+        const uint64_t hi = wi + 1 < 8 ? s[wi + 1] : 0u;
+        v = (uint32_t)(((lo | (hi << 32)) >> off) & 63u);
+    }
+    const int raw = (int)(v >> 1), cin = (int)(v & 1u);
+    return raw + cin - ((raw >> 4) << 5);
+}
+
+// One wavefront per (MSM k, window w): S[k * 52 + w] = sum_i T_i[d_kiw].
+template <class C>
+__global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__restrict__ tables,
+                                                          const uint32_t *__restrict__ scal, uint32_t n,
+                                                          xyzz *__restrict__ S)
+{
+    using F = typename C::F;
+    __shared__ xyzz sh[64];
+    const uint32_t k = blockIdx.x / FIXED_WINDOWS;
+    const int w = (int)(blockIdx.x % FIXED_WINDOWS);
+    const uint32_t *sk = scal + (size_t)k * n * 8;
+    xyzz acc;
+    bool acc_id = true;
+    xyzz_identity<F>(acc);
+    for (uint32_t i = threadIdx.x; i < n; i += 64)
+    {
+        const int d = fixed_digit(sk + 8ull * i, w);
+        if (d == 0)
+            continue;
+        aff q;
+        load_aff(q, tables, i * FIXED_ENTRIES + (uint32_t)((d < 0 ? -d : d) - 1));
+        if (fe_is_zero<F>(q.x) && fe_is_zero<F>(q.y))
+            continue; // identity base
+        if (d < 0)
+            fe_neg<F>(q.y, q.y);
+        xyzz_madd<F>(acc, acc_id, q);
+    }
+    if (acc_id)
+        xyzz_identity<F>(acc);
+    store_xyzz(sh + threadIdx.x, acc);
+    __syncthreads();
+    for (uint32_t h = 32; h > 0; h >>= 1)
+    {
+        if (threadIdx.x < h)
+        {
+            xyzz a, b;
+            load_xyzz(a, sh + threadIdx.x);
+            load_xyzz(b, sh + threadIdx.x + h);
+            xyzz_add<F>(a, a, b);
+            store_xyzz(sh + threadIdx.x, a);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        xyzz r;
+        load_xyzz(r, sh);
+        store_xyzz(S + blockIdx.x, r);
+    }
+}
+
+// One thread per MSM: Horner over the 52 window sums (5 shared doublings per window), then the
+// 32-byte encoding and the reference struct (affine, Z = 1; identity 1:1:0).
+template <class C>
+__global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S, uint32_t K,
+                                                     uint32_t *__restrict__ out32, uint64_t *__restrict__ outjac)
+{
+    using F = typename C::F;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K)
+        return;
+    const xyzz *sk = S + (size_t)k * FIXED_WINDOWS;
+    xyzz s;
+    jac acc, t;
+    load_xyzz(s, sk + FIXED_WINDOWS - 1);
+    xyzz_to_jac<F>(acc, s);
+    for (int w = FIXED_WINDOWS - 2; w >= 0; w--)
+    {
+        for (int j = 0; j < 5; j++)
+            jac_dbl<F>(acc, acc);
+        load_xyzz(s, sk + w);
+        xyzz_to_jac<F>(t, s);
+        jac_add<F>(acc, acc, t);
+    }
+    aff a;
+    const bool pt = jac_to_aff<F>(a, acc);
+    uint32_t enc[8];
+    encode_aff<F>(enc, a, pt);
+    if (out32)
+        for (int j = 0; j < 8; j++)
+            out32[8ull * k + j] = enc[j];
+    if (outjac)
+        store_r51_aff<F>(outjac + 15ull * k, a, pt);
+}
+
+} // namespace hs

@@ -12,6 +12,7 @@
 
 #include "../../include/hs_gpu_msm.h"
 #include "msm_kernels.h"
+#include "fixed_kernels.h"
 
 namespace
 {
@@ -652,6 +653,255 @@ void slot(int curve, hs_gpu_jacobian *result, const unsigned char *scalars, cons
 
 } // namespace
 
+// ---------------------------------------------------------------- fixed-base sets
+struct hs_gpu_fixed
+{
+    int curve = 0;
+    uint32_t n = 0;
+    DevBuf tables, points; // n x 16 canonical affine entries; n reference Jacobian structs
+    DevBuf scal, sums;     // per-call workspace: prepared scalars, window sums
+    hs_gpu_batch *pip = nullptr; // cached Pippenger plan (method HS_GPU_FIXED_PIPPENGER)
+    const void *pip_s = nullptr;
+    void *pip_o = nullptr, *pip_j = nullptr;
+    size_t pip_K = 0;
+    unsigned pip_flags = 0;
+    std::mutex mu;
+};
+
+namespace
+{
+
+// The n bases of a fixed set must fit the 32-bit table indexing and the batch plan limits.
+constexpr size_t FIXED_MAX_BASES = (size_t)1 << 24;
+
+int fixed_alloc(hs_gpu_fixed **out, int curve, size_t n)
+{
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        return fail(HS_GPU_EINVAL, "unknown curve id");
+    if (n == 0 || n >= FIXED_MAX_BASES)
+        return fail(HS_GPU_EINVAL, "fixed-base set needs 0 < n < 2^24");
+    Ctx &cx = ctx();
+    int r;
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if ((r = cx.init()))
+            return r;
+    }
+    auto *f = new hs_gpu_fixed();
+    f->curve = curve;
+    f->n = (uint32_t)n;
+    if ((r = f->tables.reserve(n * hs::FIXED_ENTRIES * 64)) || (r = f->points.reserve(n * sizeof(hs_gpu_jacobian))))
+    {
+        hs_gpu_fixed_destroy(f);
+        return r;
+    }
+    *out = f;
+    return 0;
+}
+
+template <class C> int fixed_build_tables(hs_gpu_fixed &f, uint64_t *d_ref_out, hipStream_t st)
+{
+    const uint32_t t = f.n * hs::FIXED_ENTRIES;
+    hipLaunchKernelGGL(hs::k_fixed_tables<C>, dim3((t + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
+                       f.tables.as<uint32_t>(), d_ref_out);
+    HS_CHECK(hipGetLastError());
+    return 0;
+}
+
+// count MSMs over the set; comb method (width-5 tables, shared doublings)
+template <class C>
+int fixed_comb_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, unsigned flags, void *d_out32,
+                       void *d_outjac, hipStream_t st)
+{
+    int r;
+    const size_t rows = count * f.n;
+    if ((r = f.scal.reserve(rows * 32)) || (r = f.sums.reserve(count * hs::FIXED_WINDOWS * sizeof(hs::xyzz))))
+        return r;
+    const uint32_t compat = (!(flags & HS_GPU_EXACT) && ref_window(f.n) == 8) ? 1u : 0u;
+    hipLaunchKernelGGL(hs::k_fixed_prep<C>, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st,
+                       (const uint32_t *)d_scalars, rows, f.scal.as<uint32_t>(), compat);
+    hipLaunchKernelGGL(hs::k_fixed_window_sums<C>, dim3((unsigned)(count * hs::FIXED_WINDOWS)), dim3(64), 0, st,
+                       f.tables.as<uint32_t>(), f.scal.as<uint32_t>(), f.n, f.sums.as<hs::xyzz>());
+    hipLaunchKernelGGL(hs::k_fixed_horner<C>, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st,
+                       f.sums.as<hs::xyzz>(), (uint32_t)count, (uint32_t *)d_out32, (uint64_t *)d_outjac);
+    HS_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Pippenger method: one batched plan whose MSMs all read the resident bases
+int fixed_pippenger_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, unsigned flags, void *d_out32,
+                            void *d_outjac, hipStream_t st)
+{
+    if (!f.pip || f.pip_s != d_scalars || f.pip_K != count || f.pip_o != d_out32 || f.pip_j != d_outjac ||
+        f.pip_flags != flags)
+    {
+        if (f.pip)
+        {
+            HS_CHECK(hipStreamSynchronize(st));
+            hs_gpu_batch_destroy(f.pip);
+            f.pip = nullptr;
+        }
+        std::vector<hs_gpu_msm_desc> d(count);
+        for (size_t k = 0; k < count; k++)
+        {
+            d[k].curve = f.curve;
+            d[k].n = f.n;
+            d[k].scalars = (const unsigned char *)d_scalars + k * f.n * 32;
+            d[k].points = f.points.as<hs_gpu_jacobian>();
+            d[k].out32 = d_out32 ? (unsigned char *)d_out32 + 32 * k : nullptr;
+            d[k].result = d_outjac ? (hs_gpu_jacobian *)d_outjac + k : nullptr;
+        }
+        int r;
+        if ((r = batch_create(&f.pip, d.data(), count, flags)))
+            return r;
+        f.pip_s = d_scalars;
+        f.pip_K = count;
+        f.pip_o = d_out32;
+        f.pip_j = d_outjac;
+        f.pip_flags = flags;
+    }
+    return batch_enqueue(*f.pip, st);
+}
+
+int fixed_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, unsigned flags, int method, void *d_out32,
+                  void *d_outjac, hipStream_t st)
+{
+    if (method == HS_GPU_FIXED_AUTO)
+        method = HS_GPU_FIXED_COMB;
+    if (method == HS_GPU_FIXED_PIPPENGER)
+        return fixed_pippenger_enqueue(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    if (method != HS_GPU_FIXED_COMB)
+        return fail(HS_GPU_EINVAL, "unknown fixed-base method");
+    if (count * hs::FIXED_WINDOWS >= ((size_t)1 << 31))
+        return fail(HS_GPU_EINVAL, "too many MSMs in one fixed-base call");
+    return f.curve == HS_GPU_HELIOS
+               ? fixed_comb_enqueue<hs::HeliosC>(f, d_scalars, count, flags, d_out32, d_outjac, st)
+               : fixed_comb_enqueue<hs::SeleneC>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+}
+
+// Host-buffer round trip of `count` MSMs over f (synchronous on the library stream).
+int fixed_msm_host(hs_gpu_fixed &f, const unsigned char *scalars, size_t count, unsigned flags, int method,
+                   unsigned char *out32, hs_gpu_jacobian *results)
+{
+    if (count == 0)
+        return 0;
+    if (!scalars)
+        return fail(HS_GPU_EINVAL, "null scalars");
+    Ctx &cx = ctx();
+    hipStream_t st = cx.stream;
+    DevBuf ds, dout;
+    int r;
+    if ((r = ds.reserve(count * f.n * 32)) || (r = dout.reserve(count * (32 + sizeof(hs_gpu_jacobian)))))
+    {
+        hipFree(ds.p);
+        hipFree(dout.p);
+        return r;
+    }
+    unsigned char *o32 = dout.as<unsigned char>();
+    hs_gpu_jacobian *oj = (hs_gpu_jacobian *)(o32 + count * 32);
+    hipError_t e = hipMemcpyAsync(ds.p, scalars, count * f.n * 32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        r = fixed_enqueue(f, ds.p, count, flags, method, o32, oj, st);
+    else
+        r = fail(HS_GPU_EDEVICE, std::string("scalar upload: ") + hipGetErrorString(e));
+    if (!r && out32 && hipMemcpyAsync(out32, o32, count * 32, hipMemcpyDeviceToHost, st) != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, "result copy failed");
+    if (!r && results && hipMemcpyAsync(results, oj, count * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToHost, st) != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, "result copy failed");
+    if (hipStreamSynchronize(st) != hipSuccess && !r)
+        r = fail(HS_GPU_EDEVICE, "fixed-base MSM failed");
+    if (f.pip && method == HS_GPU_FIXED_PIPPENGER)
+    {
+        hs_gpu_batch_destroy(f.pip); // its descriptors point at the buffers freed below
+        f.pip = nullptr;
+    }
+    hipFree(ds.p);
+    hipFree(dout.p);
+    return r;
+}
+
+void slot_msm_fixed(int curve, hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_affine *const *tables,
+                    size_t n)
+{
+    if (n == 0)
+    {
+        if (result)
+            set_identity(result);
+        return;
+    }
+    int r = 0;
+    hs_gpu_fixed *f = nullptr;
+    if (!scalars || !tables)
+        r = fail(HS_GPU_EINVAL, "null input with n > 0");
+    if (!r)
+        r = fixed_alloc(&f, curve, n);
+    if (!r)
+    {
+        // stage the caller's tables (n pointers to 16 reference affine entries) in one upload
+        std::vector<hs_gpu_affine> h(n * hs::FIXED_ENTRIES);
+        for (size_t i = 0; i < n && !r; i++)
+        {
+            if (!tables[i])
+                r = fail(HS_GPU_EINVAL, "null table pointer");
+            else
+                std::memcpy(&h[i * hs::FIXED_ENTRIES], tables[i], hs::FIXED_ENTRIES * sizeof(hs_gpu_affine));
+        }
+        DevBuf staged;
+        hipStream_t st = ctx().stream;
+        if (!r && !(r = staged.reserve(h.size() * sizeof(hs_gpu_affine))))
+        {
+            if (hipMemcpyAsync(staged.p, h.data(), h.size() * sizeof(hs_gpu_affine), hipMemcpyHostToDevice, st) !=
+                hipSuccess)
+                r = fail(HS_GPU_EDEVICE, "table upload failed");
+            const uint32_t entries = (uint32_t)h.size();
+            if (!r)
+            {
+                if (curve == HS_GPU_HELIOS)
+                    hipLaunchKernelGGL(hs::k_fixed_tables_from_ref<hs::HeliosC>, dim3((entries + 255) / 256), dim3(256),
+                                       0, st, staged.as<uint64_t>(), entries, f->tables.as<uint32_t>());
+                else
+                    hipLaunchKernelGGL(hs::k_fixed_tables_from_ref<hs::SeleneC>, dim3((entries + 255) / 256), dim3(256),
+                                       0, st, staged.as<uint64_t>(), entries, f->tables.as<uint32_t>());
+                r = fixed_msm_host(*f, scalars, 1, HS_GPU_EXACT, HS_GPU_FIXED_COMB, nullptr, result);
+            }
+            hipFree(staged.p);
+            staged.p = nullptr;
+        }
+    }
+    if (f)
+        hs_gpu_fixed_destroy(f);
+    if (r)
+    {
+        std::fprintf(stderr, "hs_gpu %s_msm_fixed failed (%d): %s\n", curve == 0 ? "helios" : "selene", r, g_err.c_str());
+        if (result)
+            set_identity(result);
+    }
+}
+
+void slot_pedersen(int curve, hs_gpu_jacobian *result, const unsigned char *blinding, const hs_gpu_jacobian *H,
+                   const unsigned char *values, const hs_gpu_jacobian *generators, size_t n)
+{
+    // guard as helios_pedersen.h:63-68, then one MSM of n + 1 terms (:70-84)
+    if (n > SIZE_MAX / 32 - 1 || !blinding || !H || (n && (!values || !generators)))
+    {
+        if (result)
+            set_identity(result);
+        return;
+    }
+    std::vector<unsigned char> s(32 * (n + 1));
+    std::vector<hs_gpu_jacobian> p(n + 1);
+    std::memcpy(s.data(), blinding, 32);
+    p[0] = *H;
+    if (n)
+    {
+        std::memcpy(s.data() + 32, values, 32 * n);
+        std::memcpy(p.data() + 1, generators, n * sizeof(hs_gpu_jacobian));
+    }
+    slot(curve, result, s.data(), p.data(), n + 1);
+}
+
+} // namespace
+
 extern "C"
 {
 
@@ -1094,4 +1344,161 @@ int hs_gpu_window_bits(int curve, size_t n)
 
 int hs_gpu_reference_window_bits(size_t n) { return ref_window(n); }
 
+int hs_gpu_fixed_create(int curve, const hs_gpu_jacobian *points, size_t n, hs_gpu_fixed **out)
+{
+    if (!out || !points)
+        return fail(HS_GPU_EINVAL, "null argument");
+    hs_gpu_fixed *f = nullptr;
+    int r;
+    if ((r = fixed_alloc(&f, curve, n)))
+        return r;
+    hipStream_t st = ctx().stream;
+    if (hipMemcpyAsync(f->points.p, points, n * sizeof(hs_gpu_jacobian), hipMemcpyHostToDevice, st) != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, "base upload failed");
+    if (!r)
+        r = curve == HS_GPU_HELIOS ? fixed_build_tables<hs::HeliosC>(*f, nullptr, st)
+                                   : fixed_build_tables<hs::SeleneC>(*f, nullptr, st);
+    if (!r && hipStreamSynchronize(st) != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, "table build failed");
+    if (r)
+    {
+        hs_gpu_fixed_destroy(f);
+        return r;
+    }
+    *out = f;
+    return 0;
+}
+
+int hs_gpu_fixed_create_device(int curve, const void *d_points, size_t n, void *stream, hs_gpu_fixed **out)
+{
+    if (!out || !d_points)
+        return fail(HS_GPU_EINVAL, "null argument");
+    hs_gpu_fixed *f = nullptr;
+    int r;
+    if ((r = fixed_alloc(&f, curve, n)))
+        return r;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx().stream;
+    if (hipMemcpyAsync(f->points.p, d_points, n * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToDevice, st) != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, "base copy failed");
+    if (!r)
+        r = curve == HS_GPU_HELIOS ? fixed_build_tables<hs::HeliosC>(*f, nullptr, st)
+                                   : fixed_build_tables<hs::SeleneC>(*f, nullptr, st);
+    if (!r && hipStreamSynchronize(st) != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, "table build failed");
+    if (r)
+    {
+        hs_gpu_fixed_destroy(f);
+        return r;
+    }
+    *out = f;
+    return 0;
+}
+
+size_t hs_gpu_fixed_size(const hs_gpu_fixed *f) { return f ? f->n : 0; }
+
+int hs_gpu_fixed_msm_device(hs_gpu_fixed *f, const void *d_scalars, size_t count, unsigned flags, int method,
+                            void *d_out32, void *d_out_jac, void *stream)
+{
+    if (!f || (count && !d_scalars))
+        return fail(HS_GPU_EINVAL, "null argument");
+    if (count == 0)
+        return 0;
+    std::lock_guard<std::mutex> lk(f->mu);
+    return fixed_enqueue(*f, d_scalars, count, flags, method, d_out32, d_out_jac,
+                         stream ? (hipStream_t)stream : ctx().stream);
+}
+
+int hs_gpu_fixed_msm(hs_gpu_fixed *f, const unsigned char *scalars, size_t count, unsigned flags, int method,
+                     unsigned char *out32, hs_gpu_jacobian *results)
+{
+    if (!f)
+        return fail(HS_GPU_EINVAL, "null fixed-base set");
+    std::lock_guard<std::mutex> lk(f->mu);
+    return fixed_msm_host(*f, scalars, count, flags, method, out32, results);
+}
+
+int hs_gpu_pedersen_commit_batch(hs_gpu_fixed *f, const unsigned char *blindings, const unsigned char *values,
+                                 size_t count, unsigned flags, int method, unsigned char *out32, hs_gpu_jacobian *results)
+{
+    if (!f || (count && (!blindings || (f->n > 1 && !values))))
+        return fail(HS_GPU_EINVAL, "null argument");
+    const size_t nv = f->n - 1;
+    std::vector<unsigned char> rows(count * f->n * 32);
+    for (size_t k = 0; k < count; k++)
+    {
+        std::memcpy(&rows[k * f->n * 32], blindings + 32 * k, 32);
+        if (nv)
+            std::memcpy(&rows[k * f->n * 32 + 32], values + k * nv * 32, nv * 32);
+    }
+    std::lock_guard<std::mutex> lk(f->mu);
+    return fixed_msm_host(*f, rows.data(), count, flags, method, out32, results);
+}
+
+void hs_gpu_fixed_destroy(hs_gpu_fixed *f)
+{
+    if (!f)
+        return;
+    if (f->pip)
+        hs_gpu_batch_destroy(f->pip);
+    for (DevBuf *d : {&f->tables, &f->points, &f->scal, &f->sums})
+        if (d->p)
+            hipFree(d->p);
+    delete f;
+}
+
+int hs_gpu_fixed_precompute(int curve, const hs_gpu_jacobian *points, size_t n, hs_gpu_affine *tables)
+{
+    if (n == 0)
+        return 0;
+    if (!points || !tables)
+        return fail(HS_GPU_EINVAL, "null argument");
+    hs_gpu_fixed *f = nullptr;
+    int r;
+    if ((r = fixed_alloc(&f, curve, n)))
+        return r;
+    DevBuf ref;
+    hipStream_t st = ctx().stream;
+    const size_t bytes = n * hs::FIXED_ENTRIES * sizeof(hs_gpu_affine);
+    if (!(r = ref.reserve(bytes)))
+    {
+        if (hipMemcpyAsync(f->points.p, points, n * sizeof(hs_gpu_jacobian), hipMemcpyHostToDevice, st) != hipSuccess)
+            r = fail(HS_GPU_EDEVICE, "base upload failed");
+        if (!r)
+            r = curve == HS_GPU_HELIOS ? fixed_build_tables<hs::HeliosC>(*f, ref.as<uint64_t>(), st)
+                                       : fixed_build_tables<hs::SeleneC>(*f, ref.as<uint64_t>(), st);
+        if (!r && hipMemcpyAsync(tables, ref.p, bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
+            r = fail(HS_GPU_EDEVICE, "table copy failed");
+        if (!r && hipStreamSynchronize(st) != hipSuccess)
+            r = fail(HS_GPU_EDEVICE, "table build failed");
+        hipFree(ref.p);
+    }
+    hs_gpu_fixed_destroy(f);
+    return r;
+}
+
+void hs_gpu_helios_msm_fixed(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_affine *const *tables,
+                             size_t n)
+{
+    slot_msm_fixed(HS_GPU_HELIOS, result, scalars, tables, n);
+}
+
+void hs_gpu_selene_msm_fixed(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_affine *const *tables,
+                             size_t n)
+{
+    slot_msm_fixed(HS_GPU_SELENE, result, scalars, tables, n);
+}
+
+void hs_gpu_helios_pedersen_commit(hs_gpu_jacobian *result, const unsigned char *blinding, const hs_gpu_jacobian *H,
+                                   const unsigned char *values, const hs_gpu_jacobian *generators, size_t n)
+{
+    slot_pedersen(HS_GPU_HELIOS, result, blinding, H, values, generators, n);
+}
+
+void hs_gpu_selene_pedersen_commit(hs_gpu_jacobian *result, const unsigned char *blinding, const hs_gpu_jacobian *H,
+                                   const unsigned char *values, const hs_gpu_jacobian *generators, size_t n)
+{
+    slot_pedersen(HS_GPU_SELENE, result, blinding, H, values, generators, n);
+}
+
 } // extern "C"
+

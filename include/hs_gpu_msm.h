@@ -119,6 +119,63 @@ int hs_gpu_batch_capture(hs_gpu_batch *batch);
 size_t hs_gpu_batch_points(const hs_gpu_batch *batch);
 void hs_gpu_batch_destroy(hs_gpu_batch *batch);
 
+/* ---------------------------------------------------------------- Pedersen and fixed-base MSM
+ * Layout-identical to the reference's helios_affine / selene_affine (x, y as radix-2^51 limbs,
+ * 80 bytes; helios/include/helios.h:48-52). */
+typedef struct hs_gpu_affine
+{
+    uint64_t x[5];
+    uint64_t y[5];
+} hs_gpu_affine;
+
+/* Drop-in for helios_pedersen_commit / selene_pedersen_commit
+ * (helios/include/helios_pedersen.h:55-86): blinding * H + sum values[i] * generators[i], one MSM
+ * of n + 1 terms with the plain MSM's result semantics; n > SIZE_MAX/32 - 1 gives the identity. */
+void hs_gpu_helios_pedersen_commit(hs_gpu_jacobian *result, const unsigned char *blinding, const hs_gpu_jacobian *H,
+                                   const unsigned char *values, const hs_gpu_jacobian *generators, size_t n);
+void hs_gpu_selene_pedersen_commit(hs_gpu_jacobian *result, const unsigned char *blinding, const hs_gpu_jacobian *H,
+                                   const unsigned char *values, const hs_gpu_jacobian *generators, size_t n);
+
+/* Width-5 fixed-base tables [1P .. 16P] for n bases, 16 entries per base written to
+ * tables[16 * i + k] = (k + 1) * P_i (canonical limbs); the n-base form of
+ * helios_scalarmult_fixed_precompute (helios/include/helios_scalarmult_fixed.h:54-65).  An identity
+ * base gets sixteen (0, 0) entries. */
+int hs_gpu_fixed_precompute(int curve, const hs_gpu_jacobian *points, size_t n, hs_gpu_affine *tables);
+
+/* Drop-in for helios_msm_fixed / selene_msm_fixed (helios/include/helios_msm_fixed.h:60-61):
+ * sum scalars[i] * P_i from the callers' 16-entry tables.  Exact sum (the reference's mixed
+ * additions assume the accumulator never equals +-the table entry; this one has no such case). */
+void hs_gpu_helios_msm_fixed(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_affine *const *tables,
+                             size_t n);
+void hs_gpu_selene_msm_fixed(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_affine *const *tables,
+                             size_t n);
+
+/* Resident fixed-base sets (no reference counterpart: the reference recomputes per call).  A set
+ * keeps n bases and their width-5 tables in HBM; each MSM over the set takes one row of n scalars
+ * (n x 32 bytes) and has the plain MSM's result semantics (HS_GPU_EXACT as there).  method:
+ * HS_GPU_FIXED_COMB    the tables with 52 shared windows of 5 doublings (helios_msm_fixed's
+ *                      algorithm), one wavefront per (MSM, window);
+ * HS_GPU_FIXED_PIPPENGER  the batched Pippenger over the resident bases;
+ * HS_GPU_FIXED_AUTO    the faster of the two for this shape.
+ * A Pedersen set holds [H, G_0 .. G_{m-1}] (n = m + 1); hs_gpu_pedersen_commit_batch takes
+ * count blindings (count x 32 bytes) and count value vectors (count x m x 32 bytes).  Calls on one
+ * set are serialised; the device form is asynchronous on `stream`. */
+#define HS_GPU_FIXED_AUTO 0
+#define HS_GPU_FIXED_COMB 1
+#define HS_GPU_FIXED_PIPPENGER 2
+typedef struct hs_gpu_fixed hs_gpu_fixed;
+int hs_gpu_fixed_create(int curve, const hs_gpu_jacobian *points, size_t n, hs_gpu_fixed **out);
+int hs_gpu_fixed_create_device(int curve, const void *d_points, size_t n, void *stream, hs_gpu_fixed **out);
+size_t hs_gpu_fixed_size(const hs_gpu_fixed *set);
+int hs_gpu_fixed_msm(hs_gpu_fixed *set, const unsigned char *scalars, size_t count, unsigned flags, int method,
+                     unsigned char *out32, hs_gpu_jacobian *results);
+int hs_gpu_fixed_msm_device(hs_gpu_fixed *set, const void *d_scalars, size_t count, unsigned flags, int method,
+                            void *d_out32, void *d_out_jac, void *stream);
+int hs_gpu_pedersen_commit_batch(hs_gpu_fixed *set, const unsigned char *blindings, const unsigned char *values,
+                                 size_t count, unsigned flags, int method, unsigned char *out32,
+                                 hs_gpu_jacobian *results);
+void hs_gpu_fixed_destroy(hs_gpu_fixed *set);
+
 /* ---------------------------------------------------------------- hash-to-curve
  * Simplified SWU map of n 32-byte inputs (RFC 9380 6.6.2; reference helios_map_to_curve /
  * selene_map_to_curve, helios/src/x64/helios_map_to_curve.cpp:184-189).  Outputs the reference

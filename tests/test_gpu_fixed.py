@@ -1,0 +1,142 @@
+"""GPU parity tests for Pedersen commitments and the fixed-base (width-5 table) MSM.
+
+Checked against the reference's golden pedersen_commit vectors, the oracle's restatements of
+helios_pedersen_commit / helios_msm_fixed (oracle/hs_oracle.c), the oracle's scalar multiple for
+every table entry, and the plain GPU MSM (itself pinned against the oracle and the reference).
+"""
+import random
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import gpu_helpers as GH
+
+hs = pytest.importorskip("helioselene_amd")
+pytestmark = pytest.mark.gpu
+H = bytes.fromhex
+
+
+def _affine_entry_bytes(c, e):
+    """reference affine struct (x[5], y[5]) -> compressed bytes; (0, 0) -> 0^32."""
+    if not e.any():
+        return bytes(32)
+    jac = np.zeros(15, dtype=np.uint64)
+    jac[:10] = e
+    jac[10] = 1
+    return GH.jac_to_bytes(c, jac)
+
+
+@pytest.mark.parametrize("c", [0, 1])
+def test_precompute_tables_are_multiples(c):
+    n = 24
+    pb = bytearray(O.gen_sswu_bases(c, 11 + c, n))
+    pb[32 * 5:32 * 6] = bytes(32)  # one identity base
+    pts = GH.to_jac_array(c, bytes(pb), n, rnd=random.Random(c))
+    tab = hs.fixed_precompute(c, pts)
+    assert tab.shape == (n, 16, 10)
+    for i in range(n):
+        P = bytes(pb[32 * i:32 * i + 32])
+        for k in range(16):
+            got = _affine_entry_bytes(c, tab[i, k])
+            want = bytes(32) if P == bytes(32) else O.scalar_mul(c, (k + 1).to_bytes(32, "little"), P)
+            assert got == want, (i, k)
+
+
+@pytest.mark.parametrize("c", [0, 1])
+@pytest.mark.parametrize("n", [1, 2, 7, 100, 1000])
+def test_msm_fixed_slot_matches_oracle(c, n):
+    sc = bytearray(O.gen_scalars(c, 40 + n, n, edge_frac=0.1 if n > 10 else 0.0))
+    pb = bytearray(O.gen_sswu_bases(c, 50 + n, n, dup_frac=0.05 if n > 20 else 0.0))
+    if n > 3:
+        pb[32:64] = bytes(32)
+    pts = GH.to_jac_array(c, bytes(pb), n)
+    tab = hs.fixed_precompute(c, pts)
+    res = hs.msm_fixed(c, GH.scalars_np(bytes(sc), n), tab)
+    assert GH.jac_to_bytes(c, res) == O.msm_naive(c, bytes(sc), bytes(pb), n)
+
+
+def test_msm_fixed_slot_empty():
+    res = hs.msm_fixed(0, np.zeros((0, 32), np.uint8), np.zeros((0, 16, 10), np.uint64))
+    assert GH.jac_to_bytes(0, res) == bytes(32)
+
+
+@pytest.mark.parametrize("c", [0, 1])
+def test_pedersen_slot_golden(golden, c):
+    v = golden[0][O.CURVES[c] + "_point"]
+    for t in v["pedersen_commit"]:
+        n = t["n"]
+        Hp = GH.to_jac_array(c, H(t["H"]), 1)[0]
+        gens = GH.to_jac_array(c, b"".join(H(x) for x in t["generators"]), n) if n else np.zeros((0, 15), np.uint64)
+        vals = GH.scalars_np(b"".join(H(x) for x in t["values"]), n) if n else np.zeros((0, 32), np.uint8)
+        res = hs.pedersen_commit(c, H(t["blinding"]), Hp, vals, gens)
+        assert GH.jac_to_bytes(c, res).hex() == t["result"], t["label"]
+
+
+@pytest.mark.parametrize("c", [0, 1])
+@pytest.mark.parametrize("n", [0, 5, 999])
+def test_pedersen_slot_matches_oracle(c, n):
+    """n = 999 puts the n + 1 term MSM in the reference's window-8 band."""
+    sc = O.gen_scalars(c, 70 + n, n + 1)
+    pb = O.gen_sswu_bases(c, 80 + n, n + 1)
+    pts = GH.to_jac_array(c, pb, n + 1)
+    res = hs.pedersen_commit(c, sc[:32], pts[0], GH.scalars_np(sc[32:], n) if n else np.zeros((0, 32), np.uint8),
+                             pts[1:])
+    want = O.pedersen(c, sc[:32], pb[:32], sc[32:], pb[32:], n)
+    assert GH.jac_to_bytes(c, res) == want
+
+
+@pytest.mark.parametrize("c", [0, 1])
+@pytest.mark.parametrize("method", [hs.FIXED_COMB, hs.FIXED_PIPPENGER])
+def test_fixed_set_rows_match_oracle(c, method):
+    n, K = 300, 5
+    pb = bytearray(O.gen_sswu_bases(c, 90 + c, n, dup_frac=0.02))
+    pb[32 * 7:32 * 8] = bytes(32)
+    pts = GH.to_jac_array(c, bytes(pb), n)
+    rows = [O.gen_scalars(c, 100 + k, n, edge_frac=0.05) for k in range(K)]
+    rows[1] = bytes(32 * n)  # all-zero row: identity
+    fs = hs.FixedSet(c, pts)
+    out, jac = fs.msm(np.frombuffer(b"".join(rows), np.uint8).reshape(K, n, 32), method=method)
+    fs.close()
+    for k in range(K):
+        want = O.msm(c, rows[k], bytes(pb), n)
+        assert out[k] == want, k
+        assert GH.jac_to_bytes(c, jac[k]) == want
+
+
+@pytest.mark.parametrize("method", [hs.FIXED_COMB, hs.FIXED_PIPPENGER])
+def test_fixed_set_window8_band(method):
+    """A 1000-base set: default rows reproduce the reference's MSM bytes, exact rows the sum."""
+    c, n, K = 0, 1000, 3
+    pb = O.gen_sswu_bases(c, 123, n)
+    pts = GH.to_jac_array(c, pb, n)
+    rows = [O.gen_scalars(c, 200 + k, n) for k in range(K)]
+    S = np.frombuffer(b"".join(rows), np.uint8).reshape(K, n, 32)
+    fs = hs.FixedSet(c, pts)
+    compat, _ = fs.msm(S, method=method)
+    exact, _ = fs.msm(S, flags=hs.EXACT, method=method)
+    fs.close()
+    for k in range(K):
+        assert compat[k] == O.msm(c, rows[k], pb, n)
+        assert exact[k] == O.msm_naive(c, rows[k], pb, n)
+
+
+@pytest.mark.parametrize("c", [0, 1])
+def test_pedersen_batch_4096_generators(c):
+    """The commitment shape of the Pedersen workload (H + 2^12 generators), both methods against
+    the plain MSM of each row."""
+    m, K = 4096, 3
+    rng = np.random.default_rng(c)
+    pts, _ = hs.map_to_curve(c, rng.integers(0, 256, size=(m + 1) * 32, dtype=np.uint8).tobytes())
+    blind = rng.integers(0, 256, size=(K, 32), dtype=np.uint8)
+    vals = rng.integers(0, 256, size=(K, m, 32), dtype=np.uint8)
+    blind[:, 31] &= 0x7F
+    vals[:, :, 31] &= 0x7F
+    fs = hs.FixedSet(c, pts)
+    comb, _ = fs.pedersen(blind, vals, method=hs.FIXED_COMB)
+    pip, _ = fs.pedersen(blind, vals, method=hs.FIXED_PIPPENGER)
+    fs.close()
+    for k in range(K):
+        row = np.concatenate([blind[k:k + 1], vals[k]], axis=0)
+        want = hs.msm(c, row, pts)[0]
+        assert comb[k] == want and pip[k] == want, k
