@@ -168,6 +168,50 @@ def run_mix(hs, torch, args):
     }
 
 
+def run_pedersen(hs, torch, args, count=1024, m=4096):
+    """Batched Pedersen commitments r*H + sum v_i G_i over one resident set per curve of H and
+    m = 2^12 generators (seeded SSWU), count commitments per curve, both fixed-base methods."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.Stream()
+    rng = np.random.default_rng(args.seed + 2000)
+    res = {"workload": f"{count} Pedersen commitments per curve over H + {m} resident generators",
+           "commitments_per_curve": count, "terms_per_commitment": m + 1}
+    with torch.cuda.stream(st):
+        sh = st.cuda_stream
+        outs = {}
+        for c in (0, 1):
+            u = torch.from_numpy(rng.integers(0, 256, size=(m + 1, 32), dtype=np.uint8)).to(dev)
+            pts = torch.empty((m + 1, 15), dtype=torch.int64, device=dev)
+            hs.map_to_curve_device(c, u.data_ptr(), m + 1, pts.data_ptr(), None, sh)
+            torch.cuda.synchronize()
+            fs = hs.FixedSet(c, d_points=pts.data_ptr(), n=m + 1, stream=sh)
+            s = rng.integers(0, 256, size=(count, m + 1, 32), dtype=np.uint8)
+            s[:, :, 31] &= 0x7F
+            ds = torch.from_numpy(s).to(dev)
+            for method, name in ((hs.FIXED_COMB, "comb_w5"), (hs.FIXED_PIPPENGER, "pippenger")):
+                out = torch.zeros((count, 8), dtype=torch.int32, device=dev)
+                fs.msm_device(ds.data_ptr(), count, out.data_ptr(), None, 0, method, sh)
+                torch.cuda.synchronize()
+                t = []
+                for _ in range(args.pedersen_steps):
+                    t0 = time.perf_counter()
+                    fs.msm_device(ds.data_ptr(), count, out.data_ptr(), None, 0, method, sh)
+                    torch.cuda.synchronize()
+                    t.append(time.perf_counter() - t0)
+                tm = float(np.median(t))
+                res[f"{['helios', 'selene'][c]}_{name}_ms"] = tm * 1e3
+                res[f"{['helios', 'selene'][c]}_{name}_points_per_s"] = count * (m + 1) / tm
+                outs[(c, name)] = out.cpu().numpy()
+            res[f"{['helios', 'selene'][c]}_methods_agree"] = bool((outs[(c, "comb_w5")] == outs[(c, "pippenger")]).all())
+            fs.close()
+            del ds
+    best = sum(min(res[f"{cv}_comb_w5_ms"], res[f"{cv}_pippenger_ms"]) for cv in ("helios", "selene"))
+    res["points_per_s_best_method"] = 2 * count * (m + 1) / (best * 1e-3)
+    return res
+
+
 def cpu_baseline(hs_pts_cpu, sc_cpu, n_sample):
     """The reference CPU MSM (x64 Pippenger behind its runtime dispatch) on the first n_sample
     inputs of each curve, one thread."""
@@ -208,6 +252,8 @@ def main():
     ap.add_argument("--mix-steps", type=int, default=20, help="timed replays of the batched mix")
     ap.add_argument("--mix-plain-steps", type=int, default=3, help="timed plain passes over the mix")
     ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--pedersen-steps", type=int, default=3)
+    ap.add_argument("--no-pedersen", action="store_true")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams")
     args = ap.parse_args()
@@ -298,6 +344,7 @@ def main():
                                    for c in (0, 1)},
     }
     rec["batched_mix"] = None if args.no_mix else run_mix(hs, torch, args)
+    rec["pedersen_batch"] = None if args.no_pedersen else run_pedersen(hs, torch, args)
     if not args.no_cpu:
         pts_cpu = [inputs[c][1].cpu().numpy().view(np.uint64) for c in (0, 1)]
         sc_cpu = [inputs[c][0].cpu().numpy() for c in (0, 1)]
