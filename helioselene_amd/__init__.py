@@ -85,6 +85,12 @@ def lib():
         L.hs_gpu_fixed_msm_device.argtypes = [vp, vp, sz, ctypes.c_uint, ctypes.c_int, vp, vp, vp]
         L.hs_gpu_pedersen_commit_batch.argtypes = [vp, vp, vp, sz, ctypes.c_uint, ctypes.c_int, vp, vp]
         L.hs_gpu_fixed_destroy.argtypes = [vp]
+        L.hs_gpu_server_create.argtypes = [vp, ctypes.POINTER(vp)]
+        L.hs_gpu_server_submit.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+        L.hs_gpu_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int]
+        L.hs_gpu_server_get_stats.argtypes = [vp, vp]
+        L.hs_gpu_server_reset_stats.argtypes = [vp]
+        L.hs_gpu_server_destroy.argtypes = [vp]
         L.hs_gpu_debug_dump.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
@@ -405,6 +411,77 @@ class FixedSet:
     def close(self):
         if self.handle:
             lib().hs_gpu_fixed_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- serving loop
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device_bytes", ctypes.c_size_t), ("lanes", ctypes.c_int), ("max_batch", ctypes.c_int),
+                ("flags", ctypes.c_uint)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("points", ctypes.c_uint64),
+                ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double), ("max_us", ctypes.c_double),
+                ("mean_us", ctypes.c_double), ("elapsed_s", ctypes.c_double)]
+
+
+class Server:
+    """The native serving loop (hs_gpu_server_*): submit from any thread, wait on tickets."""
+
+    def __init__(self, device_bytes=0, lanes=0, max_batch=0, flags=0):
+        cfg = ServerConfig(device_bytes, lanes, max_batch, flags)
+        h = ctypes.c_void_p()
+        _check(lib().hs_gpu_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._live = {}
+
+    def submit(self, curve, scalars=None, points=None, d_scalars=None, d_points=None, n=None):
+        """Host arrays (scalars uint8 (n, 32), points uint64 (n, 15)) or device addresses with n."""
+        out = np.zeros(32, dtype=np.uint8)
+        jac = np.zeros(15, dtype=np.uint64)
+        if d_scalars is not None:
+            keep = ()
+            d = MsmDesc(CURVES[curve], n, d_scalars, d_points, jac.ctypes.data, out.ctypes.data)
+            dev = 1
+        else:
+            s = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1, 32)
+            p = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1, 15)
+            keep = (s, p)
+            d = MsmDesc(CURVES[curve], p.shape[0], s.ctypes.data, p.ctypes.data, jac.ctypes.data, out.ctypes.data)
+            dev = 0
+        t = ctypes.c_uint64()
+        _check(lib().hs_gpu_server_submit(self.handle, ctypes.byref(d), dev, ctypes.byref(t)))
+        self._live[t.value] = (keep, out, jac)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """(32-byte result, jacobian uint64[15]); None on timeout."""
+        rc = lib().hs_gpu_server_wait(self.handle, ticket, timeout_ms)
+        if rc == 1:
+            return None
+        _, out, jac = self._live.pop(ticket)
+        if rc != 0:
+            raise HsGpuError(f"hs_gpu request {ticket} failed with status {rc}")
+        return out.tobytes(), jac
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().hs_gpu_server_get_stats(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().hs_gpu_server_reset_stats(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().hs_gpu_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

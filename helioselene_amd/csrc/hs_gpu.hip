@@ -6,7 +6,12 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -359,6 +364,21 @@ int choose_c_batch(size_t n)
     return c > 13 ? 13 : c;
 }
 
+// Bump allocator over one device block (the serving loop's per-lane memory).
+struct Arena
+{
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t bytes)
+    {
+        const size_t a = (used + 255) & ~(size_t)255;
+        if (a + bytes > cap)
+            return nullptr;
+        used = a + bytes;
+        return base + a;
+    }
+};
+
 struct BatchGroup
 {
     int curve = 0, c = 0, W = 0;
@@ -375,6 +395,8 @@ struct hs_gpu_batch
 {
     std::vector<BatchGroup *> groups;
     unsigned flags = 0;
+    bool in_arena = false;   // device buffers carved from a caller's arena (not freed here)
+    bool borrowed = false;   // worker streams and events belong to the caller
     std::vector<hipStream_t> workers;
     std::vector<hipEvent_t> ev;
     hipStream_t cap_stream = nullptr;
@@ -383,6 +405,7 @@ struct hs_gpu_batch
     size_t total_points = 0;
     uint32_t n_empty = 0; // MSMs with n = 0: identity outputs
     DevBuf empty;
+    std::vector<hs::BatchDesc> hempty;
 };
 
 namespace
@@ -446,7 +469,12 @@ int batch_enqueue(hs_gpu_batch &b, hipStream_t st)
     return 0;
 }
 
-int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
+// Plans a batch.  With `arena`, every device buffer is carved from it and descriptors are
+// uploaded asynchronously on `upload` (their host copies live in the plan); with `workers`
+// (and `events`, one more than workers) the plan runs on the caller's streams.
+int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count, unsigned flags, Arena *arena = nullptr,
+                 hipStream_t upload = nullptr, const std::vector<hipStream_t> *workers = nullptr,
+                 const std::vector<hipEvent_t> *events = nullptr)
 {
     Ctx &cx = ctx();
     int r;
@@ -457,6 +485,7 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
     }
     auto *b = new hs_gpu_batch();
     b->flags = flags;
+    b->in_arena = arena != nullptr;
     // group by (curve, window width)
     std::vector<std::pair<int, int>> keys;
     std::vector<hs::BatchDesc> empties;
@@ -465,7 +494,7 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
         const hs_gpu_msm_desc &d = descs[i];
         if ((d.curve != HS_GPU_HELIOS && d.curve != HS_GPU_SELENE) || (d.n && (!d.scalars || !d.points)))
         {
-            delete b;
+            hs_gpu_batch_destroy(b);
             return fail(HS_GPU_EINVAL, "bad batch descriptor " + std::to_string(i));
         }
         if (d.n == 0)
@@ -519,28 +548,73 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
         g->nseg = g->nb / g->G;
         g->chunks = ((size_t)g->N * g->W + g->L - 1) / g->L;
         const size_t E = (size_t)g->N * g->W;
-        if ((r = g->desc.reserve(g->M * sizeof(hs::BatchDesc))) || (r = g->aff.reserve((size_t)g->N * 64)) ||
-            (r = g->isid.reserve(g->N)) || (r = g->digits.reserve(E * 2)) || (r = g->sorted.reserve(E * 4 + 4)) ||
-            (r = g->offsets.reserve(((size_t)g->nb + 1) * 4)) || (r = g->buckets.reserve((size_t)g->nb * sizeof(hs::xyzz))) ||
-            (r = g->seg.reserve((size_t)g->nseg * sizeof(hs::xyzz))) || (r = g->win.reserve((size_t)g->V * sizeof(hs::xyzz))) ||
-            (r = g->total.reserve((size_t)g->M * sizeof(hs::jac))) || (r = g->chunk_bucket.reserve(g->chunks * 4 + 4)) ||
-            (r = g->bnd.reserve(2 * g->chunks * sizeof(hs::xyzz))) || (r = g->long_list.reserve((size_t)g->nb * 4)) ||
-            (r = g->long_count.reserve(64)))
+        const std::pair<DevBuf *, size_t> bufs[] = {
+            {&g->desc, g->M * sizeof(hs::BatchDesc)},
+            {&g->aff, (size_t)g->N * 64},
+            {&g->isid, g->N},
+            {&g->digits, E * 2},
+            {&g->sorted, E * 4 + 4},
+            {&g->offsets, ((size_t)g->nb + 1) * 4},
+            {&g->buckets, (size_t)g->nb * sizeof(hs::xyzz)},
+            {&g->seg, (size_t)g->nseg * sizeof(hs::xyzz)},
+            {&g->win, (size_t)g->V * sizeof(hs::xyzz)},
+            {&g->total, (size_t)g->M * sizeof(hs::jac)},
+            {&g->chunk_bucket, g->chunks * 4 + 4},
+            {&g->bnd, 2 * g->chunks * sizeof(hs::xyzz)},
+            {&g->long_list, (size_t)g->nb * 4},
+            {&g->long_count, 64}};
+        for (const auto &bf : bufs)
         {
-            hs_gpu_batch_destroy(b);
-            return r;
+            if (arena)
+            {
+                bf.first->p = arena->take(bf.second);
+                bf.first->cap = bf.second;
+                r = bf.first->p ? 0 : fail(HS_GPU_ENOMEM, "batch does not fit its device arena");
+            }
+            else
+                r = bf.first->reserve(bf.second);
+            if (r)
+            {
+                hs_gpu_batch_destroy(b);
+                return r;
+            }
         }
-        HS_CHECK(hipMemcpy(g->desc.p, g->hdesc.data(), g->M * sizeof(hs::BatchDesc), hipMemcpyHostToDevice));
+        if (upload)
+            HS_CHECK(hipMemcpyAsync(g->desc.p, g->hdesc.data(), g->M * sizeof(hs::BatchDesc), hipMemcpyHostToDevice, upload));
+        else
+            HS_CHECK(hipMemcpy(g->desc.p, g->hdesc.data(), g->M * sizeof(hs::BatchDesc), hipMemcpyHostToDevice));
     }
     if (!empties.empty())
     {
         b->n_empty = (uint32_t)empties.size();
-        if ((r = b->empty.reserve(empties.size() * sizeof(hs::BatchDesc))))
+        b->hempty = empties;
+        const size_t eb = empties.size() * sizeof(hs::BatchDesc);
+        if (arena)
+        {
+            b->empty.p = arena->take(eb);
+            b->empty.cap = eb;
+            r = b->empty.p ? 0 : fail(HS_GPU_ENOMEM, "batch does not fit its device arena");
+        }
+        else
+            r = b->empty.reserve(eb);
+        if (r)
         {
             hs_gpu_batch_destroy(b);
             return r;
         }
-        HS_CHECK(hipMemcpy(b->empty.p, empties.data(), empties.size() * sizeof(hs::BatchDesc), hipMemcpyHostToDevice));
+        if (upload)
+            HS_CHECK(hipMemcpyAsync(b->empty.p, b->hempty.data(), eb, hipMemcpyHostToDevice, upload));
+        else
+            HS_CHECK(hipMemcpy(b->empty.p, b->hempty.data(), eb, hipMemcpyHostToDevice));
+    }
+    if (workers)
+    {
+        // the caller's streams and events (at least one worker, workers + 1 events)
+        b->borrowed = true;
+        b->workers = *workers;
+        b->ev = *events;
+        *out = b;
+        return 0;
     }
     const size_t K = b->groups.size() < 4 ? (b->groups.empty() ? 1 : b->groups.size()) : 4;
     b->workers.resize(K);
@@ -902,6 +976,388 @@ void slot_pedersen(int curve, hs_gpu_jacobian *result, const unsigned char *blin
 
 } // namespace
 
+// ---------------------------------------------------------------- serving loop
+// A scheduler thread owns `lanes` device lanes.  Each lane has a stream, worker streams, and one
+// device arena of device_bytes / lanes.  Submitted requests wait in a FIFO.  Whenever a lane is
+// idle, the scheduler admits the longest FIFO prefix whose summed footprint fits the lane's arena
+// (up to max_batch requests), stages host inputs into the arena, and plans the admitted requests
+// as one batch in that arena.  It then launches the batch and copies the outputs back to pinned
+// memory.  Requests that arrive while others run are admitted as soon as a lane frees up, and
+// lanes run concurrently on the device (continuous batching).
+struct SrvReq
+{
+    hs_gpu_msm_desc d{};
+    bool device_inputs = false;
+    size_t foot = 0;
+    int status = 1; // 1 = pending
+    std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+struct SrvLane
+{
+    hipStream_t st = nullptr;
+    std::vector<hipStream_t> workers;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t done = nullptr;
+    Arena arena;
+    unsigned char *pin_out = nullptr;
+    std::vector<SrvReq *> reqs;
+    hs_gpu_batch *plan = nullptr;
+    bool busy = false;
+};
+
+struct hs_gpu_server
+{
+    size_t lane_bytes = 0;
+    int max_batch = 256;
+    unsigned flags = 0;
+    std::vector<SrvLane> lanes;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<SrvReq *> pending;
+    std::unordered_map<uint64_t, SrvReq *> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread th;
+    std::vector<double> lat_us;
+    uint64_t points_done = 0, batches = 0;
+    std::chrono::steady_clock::time_point t0;
+};
+
+namespace
+{
+
+// Device bytes one request adds to a batch: staged inputs, outputs and its share of every batch
+// buffer (all of which grow linearly with the member MSMs), plus alignment slack.
+size_t request_footprint(size_t n, bool device_inputs)
+{
+    if (n == 0)
+        return 1024;
+    const int c = choose_c_batch(n);
+    const size_t W = (256 + c - 1) / c, B = (size_t)1 << (c - 1), G = B < 8 ? B : 8, L = 32;
+    const size_t E = n * W, nb = W * B, chunks = (E + L - 1) / L + 1;
+    size_t b = sizeof(hs::BatchDesc) + n * 64 + n + E * 2 + E * 4 + 4 + (nb + 1) * 4 + nb * sizeof(hs::xyzz) +
+               (nb / G) * sizeof(hs::xyzz) + W * sizeof(hs::xyzz) + sizeof(hs::jac) + chunks * 4 + 4 +
+               2 * chunks * sizeof(hs::xyzz) + nb * 4 + 64;
+    if (!device_inputs)
+        b += n * (32 + sizeof(hs_gpu_jacobian));
+    return b + 32 + sizeof(hs_gpu_jacobian) + 16 * 256;
+}
+
+void srv_finish(hs_gpu_server &s, SrvLane &ln, int status)
+{
+    // called with s.mu held
+    const auto now = std::chrono::steady_clock::now();
+    const size_t k = ln.reqs.size();
+    for (size_t i = 0; i < k; i++)
+    {
+        SrvReq *q = ln.reqs[i];
+        if (status == 0)
+        {
+            if (q->d.out32)
+                std::memcpy(q->d.out32, ln.pin_out + 32 * i, 32);
+            if (q->d.result)
+                std::memcpy(q->d.result, ln.pin_out + 32 * k + sizeof(hs_gpu_jacobian) * i, sizeof(hs_gpu_jacobian));
+            s.lat_us.push_back(std::chrono::duration<double, std::micro>(now - q->t_submit).count());
+            s.points_done += q->d.n;
+        }
+        q->t_done = now;
+        q->status = status;
+    }
+    ln.reqs.clear();
+    if (ln.plan)
+    {
+        hs_gpu_batch_destroy(ln.plan);
+        ln.plan = nullptr;
+    }
+    ln.busy = false;
+    s.cv_done.notify_all();
+}
+
+// Launches ln.reqs as one batch on the lane.  Called without s.mu held.
+int srv_launch(hs_gpu_server &s, SrvLane &ln)
+{
+    const size_t k = ln.reqs.size();
+    ln.arena.used = 0;
+    std::vector<hs_gpu_msm_desc> d(k);
+    int r = 0;
+    for (size_t i = 0; i < k && !r; i++)
+    {
+        const SrvReq &q = *ln.reqs[i];
+        d[i] = q.d;
+        if (q.d.n && !q.device_inputs)
+        {
+            void *ds = ln.arena.take(q.d.n * 32), *dp = ln.arena.take(q.d.n * sizeof(hs_gpu_jacobian));
+            if (!ds || !dp)
+                return fail(HS_GPU_ENOMEM, "request inputs do not fit the lane arena");
+            HS_CHECK(hipMemcpyAsync(ds, q.d.scalars, q.d.n * 32, hipMemcpyHostToDevice, ln.st));
+            HS_CHECK(hipMemcpyAsync(dp, q.d.points, q.d.n * sizeof(hs_gpu_jacobian), hipMemcpyHostToDevice, ln.st));
+            d[i].scalars = (const unsigned char *)ds;
+            d[i].points = (const hs_gpu_jacobian *)dp;
+        }
+    }
+    unsigned char *dout = (unsigned char *)ln.arena.take(k * (32 + sizeof(hs_gpu_jacobian)));
+    if (!dout)
+        return fail(HS_GPU_ENOMEM, "request outputs do not fit the lane arena");
+    for (size_t i = 0; i < k; i++)
+    {
+        d[i].out32 = dout + 32 * i;
+        d[i].result = (hs_gpu_jacobian *)(dout + 32 * k + sizeof(hs_gpu_jacobian) * i);
+    }
+    if ((r = batch_create(&ln.plan, d.data(), k, s.flags, &ln.arena, ln.st, &ln.workers, &ln.ev)))
+        return r;
+    if ((r = batch_enqueue(*ln.plan, ln.st)))
+        return r;
+    HS_CHECK(hipMemcpyAsync(ln.pin_out, dout, k * (32 + sizeof(hs_gpu_jacobian)), hipMemcpyDeviceToHost, ln.st));
+    HS_CHECK(hipEventRecord(ln.done, ln.st));
+    return 0;
+}
+
+void srv_loop(hs_gpu_server *sp)
+{
+    hs_gpu_server &s = *sp;
+    hipSetDevice(ctx().device);
+    std::unique_lock<std::mutex> lk(s.mu);
+    while (true)
+    {
+        bool any_busy = false;
+        // completions
+        for (SrvLane &ln : s.lanes)
+        {
+            if (!ln.busy)
+                continue;
+            const hipError_t e = hipEventQuery(ln.done);
+            if (e == hipSuccess)
+                srv_finish(s, ln, 0);
+            else if (e != hipErrorNotReady)
+                srv_finish(s, ln, fail(HS_GPU_EDEVICE, std::string("serving batch failed: ") + hipGetErrorString(e)));
+            else
+                any_busy = true;
+        }
+        if (s.stop && s.pending.empty() && !any_busy)
+            break;
+        // admissions
+        for (SrvLane &ln : s.lanes)
+        {
+            if (ln.busy || s.pending.empty())
+                continue;
+            size_t used = 0;
+            while (!s.pending.empty() && (int)ln.reqs.size() < s.max_batch)
+            {
+                SrvReq *q = s.pending.front();
+                if (q->foot > s.lane_bytes)
+                {
+                    // can never fit: reject rather than block the queue
+                    s.pending.pop_front();
+                    q->status = fail(HS_GPU_ENOMEM, "request footprint exceeds the per-lane device budget");
+                    q->t_done = std::chrono::steady_clock::now();
+                    s.cv_done.notify_all();
+                    continue;
+                }
+                if (used + q->foot > s.lane_bytes)
+                    break;
+                used += q->foot;
+                ln.reqs.push_back(q);
+                s.pending.pop_front();
+            }
+            if (ln.reqs.empty())
+                continue;
+            ln.busy = true;
+            s.batches++;
+            lk.unlock();
+            const int r = srv_launch(s, ln);
+            lk.lock();
+            if (r)
+            {
+                hipStreamSynchronize(ln.st);
+                srv_finish(s, ln, r);
+            }
+            else
+                any_busy = true;
+        }
+        if (any_busy)
+            s.cv_work.wait_for(lk, std::chrono::microseconds(20));
+        else if (s.pending.empty() && !s.stop)
+            s.cv_work.wait(lk);
+    }
+}
+
+} // namespace
+
+extern "C"
+{
+
+int hs_gpu_server_create(const hs_gpu_server_config *cfg, hs_gpu_server **out)
+{
+    if (!out)
+        return fail(HS_GPU_EINVAL, "null argument");
+    Ctx &cx = ctx();
+    int r;
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if ((r = cx.init()))
+            return r;
+    }
+    const int lanes = cfg && cfg->lanes > 0 ? cfg->lanes : 4;
+    const size_t bytes = cfg && cfg->device_bytes ? cfg->device_bytes : ((size_t)8 << 30);
+    auto *s = new hs_gpu_server();
+    s->max_batch = cfg && cfg->max_batch > 0 ? cfg->max_batch : 256;
+    s->flags = cfg ? cfg->flags : 0;
+    s->lane_bytes = bytes / (size_t)lanes;
+    s->lanes.resize(lanes);
+    s->t0 = std::chrono::steady_clock::now();
+    for (SrvLane &ln : s->lanes)
+    {
+        bool ok = hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) == hipSuccess &&
+                  hipMalloc((void **)&ln.arena.base, s->lane_bytes) == hipSuccess &&
+                  hipHostMalloc((void **)&ln.pin_out, (size_t)s->max_batch * (32 + sizeof(hs_gpu_jacobian)), 0) ==
+                      hipSuccess;
+        ln.arena.cap = ok ? s->lane_bytes : 0;
+        ln.workers.resize(4);
+        ln.ev.resize(5);
+        for (auto &w : ln.workers)
+            ok = ok && hipStreamCreateWithFlags(&w, hipStreamNonBlocking) == hipSuccess;
+        for (auto &e : ln.ev)
+            ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (!ok)
+        {
+            hs_gpu_server_destroy(s);
+            return fail(HS_GPU_ENOMEM, "serving lane allocation failed (device_bytes too large?)");
+        }
+    }
+    s->th = std::thread(srv_loop, s);
+    *out = s;
+    return 0;
+}
+
+int hs_gpu_server_submit(hs_gpu_server *s, const hs_gpu_msm_desc *req, int device_inputs, uint64_t *ticket)
+{
+    if (!s || !req || !ticket)
+        return fail(HS_GPU_EINVAL, "null argument");
+    if ((req->curve != HS_GPU_HELIOS && req->curve != HS_GPU_SELENE) || (req->n && (!req->scalars || !req->points)))
+        return fail(HS_GPU_EINVAL, "bad request");
+    if (req->n >= ((size_t)1 << 24))
+        return fail(HS_GPU_EINVAL, "request n >= 2^24 (use the plain MSM)");
+    auto *q = new SrvReq();
+    q->d = *req;
+    q->device_inputs = device_inputs != 0;
+    q->foot = request_footprint(req->n, q->device_inputs);
+    q->t_submit = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop)
+        {
+            delete q;
+            return fail(HS_GPU_EINVAL, "server is stopping");
+        }
+        *ticket = s->next_id++;
+        s->reqs[*ticket] = q;
+        s->pending.push_back(q);
+    }
+    s->cv_work.notify_one();
+    return 0;
+}
+
+int hs_gpu_server_wait(hs_gpu_server *s, uint64_t ticket, int timeout_ms)
+{
+    if (!s)
+        return fail(HS_GPU_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end())
+        return fail(HS_GPU_EINVAL, "unknown ticket");
+    SrvReq *q = it->second;
+    auto ready = [q] { return q->status != 1; };
+    if (timeout_ms < 0)
+        s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return 1;
+    const int st = q->status;
+    s->reqs.erase(it);
+    delete q;
+    return st;
+}
+
+int hs_gpu_server_get_stats(hs_gpu_server *s, hs_gpu_server_stats *out)
+{
+    if (!s || !out)
+        return fail(HS_GPU_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::vector<double> v = s->lat_us;
+    std::sort(v.begin(), v.end());
+    auto pct = [&v](double p) { return v.empty() ? 0.0 : v[std::min(v.size() - 1, (size_t)(p * (v.size() - 1) + 0.5))]; };
+    out->completed = v.size();
+    out->batches = s->batches;
+    out->points = s->points_done;
+    out->p50_us = pct(0.50);
+    out->p99_us = pct(0.99);
+    out->max_us = v.empty() ? 0.0 : v.back();
+    double sum = 0;
+    for (double x : v)
+        sum += x;
+    out->mean_us = v.empty() ? 0.0 : sum / v.size();
+    out->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+    return 0;
+}
+
+int hs_gpu_server_reset_stats(hs_gpu_server *s)
+{
+    if (!s)
+        return fail(HS_GPU_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat_us.clear();
+    s->points_done = 0;
+    s->batches = 0;
+    s->t0 = std::chrono::steady_clock::now();
+    return 0;
+}
+
+void hs_gpu_server_destroy(hs_gpu_server *s)
+{
+    if (!s)
+        return;
+    if (s->th.joinable())
+    {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->th.join();
+    }
+    for (SrvLane &ln : s->lanes)
+    {
+        if (ln.plan)
+            hs_gpu_batch_destroy(ln.plan);
+        if (ln.st)
+            hipStreamSynchronize(ln.st);
+        for (auto w : ln.workers)
+            if (w)
+                hipStreamDestroy(w);
+        for (auto e : ln.ev)
+            if (e)
+                hipEventDestroy(e);
+        if (ln.done)
+            hipEventDestroy(ln.done);
+        if (ln.st)
+            hipStreamDestroy(ln.st);
+        if (ln.arena.base)
+            hipFree(ln.arena.base);
+        if (ln.pin_out)
+            hipHostFree(ln.pin_out);
+    }
+    for (auto &kv : s->reqs)
+        delete kv.second;
+    delete s;
+}
+
+} // extern "C"
+
+namespace
+{
+} // namespace
+
 extern "C"
 {
 
@@ -1008,19 +1464,23 @@ void hs_gpu_batch_destroy(hs_gpu_batch *b)
         hipGraphDestroy(b->graph);
     for (BatchGroup *g : b->groups)
     {
-        for (DevBuf *d : {&g->desc, &g->aff, &g->isid, &g->digits, &g->sorted, &g->offsets, &g->buckets, &g->seg, &g->win,
-                          &g->total, &g->chunk_bucket, &g->bnd, &g->long_list, &g->long_count})
-            if (d->p)
-                hipFree(d->p);
+        if (!b->in_arena)
+            for (DevBuf *d : {&g->desc, &g->aff, &g->isid, &g->digits, &g->sorted, &g->offsets, &g->buckets, &g->seg,
+                              &g->win, &g->total, &g->chunk_bucket, &g->bnd, &g->long_list, &g->long_count})
+                if (d->p)
+                    hipFree(d->p);
         delete g;
     }
-    for (hipStream_t s : b->workers)
-        hipStreamDestroy(s);
-    for (hipEvent_t e : b->ev)
-        hipEventDestroy(e);
+    if (!b->borrowed)
+    {
+        for (hipStream_t s : b->workers)
+            hipStreamDestroy(s);
+        for (hipEvent_t e : b->ev)
+            hipEventDestroy(e);
+    }
     if (b->cap_stream)
         hipStreamDestroy(b->cap_stream);
-    if (b->empty.p)
+    if (b->empty.p && !b->in_arena)
         hipFree(b->empty.p);
     delete b;
 }

@@ -176,6 +176,41 @@ int hs_gpu_pedersen_commit_batch(hs_gpu_fixed *set, const unsigned char *blindin
                                  hs_gpu_jacobian *results);
 void hs_gpu_fixed_destroy(hs_gpu_fixed *set);
 
+/* ---------------------------------------------------------------- serving loop
+ * No reference counterpart (the reference has no request loop).  A server owns `lanes` device
+ * lanes (a stream plus device_bytes / lanes of HBM each) and a scheduler thread.  Requests are
+ * plain MSM descriptors.  With device_inputs = 0 the scalars/points are host buffers that must
+ * stay valid until the ticket completes; with 1 they are device pointers.  out32 / result are
+ * host buffers written before the ticket completes.  The scheduler admits queued requests, in
+ * order, into any idle lane while their summed device footprint fits the lane (at most max_batch
+ * per lane).  Each lane's requests run as one batch, and lanes overlap on the device.  A request
+ * whose footprint exceeds a lane fails with HS_GPU_ENOMEM.  Results equal the plain MSM's (same
+ * flags semantics).
+ * hs_gpu_server_wait returns the request's status (0 or HS_GPU_E*), or 1 if timeout_ms (>= 0)
+ * expired first; a completed ticket is released by the wait that returns its status.
+ * Latencies are submit-to-completion, measured on the host. */
+typedef struct hs_gpu_server_config
+{
+    size_t device_bytes; /* HBM for in-flight requests, all lanes together; 0 = 8 GiB */
+    int lanes;           /* 0 = 4 */
+    int max_batch;       /* 0 = 256 */
+    unsigned flags;      /* HS_GPU_EXACT or 0 */
+} hs_gpu_server_config;
+
+typedef struct hs_gpu_server_stats
+{
+    uint64_t completed, batches, points;
+    double p50_us, p99_us, max_us, mean_us, elapsed_s;
+} hs_gpu_server_stats;
+
+typedef struct hs_gpu_server hs_gpu_server;
+int hs_gpu_server_create(const hs_gpu_server_config *config, hs_gpu_server **out);
+int hs_gpu_server_submit(hs_gpu_server *server, const hs_gpu_msm_desc *request, int device_inputs, uint64_t *ticket);
+int hs_gpu_server_wait(hs_gpu_server *server, uint64_t ticket, int timeout_ms);
+int hs_gpu_server_get_stats(hs_gpu_server *server, hs_gpu_server_stats *out);
+int hs_gpu_server_reset_stats(hs_gpu_server *server);
+void hs_gpu_server_destroy(hs_gpu_server *server); /* drains queued and running requests first */
+
 /* ---------------------------------------------------------------- hash-to-curve
  * Simplified SWU map of n 32-byte inputs (RFC 9380 6.6.2; reference helios_map_to_curve /
  * selene_map_to_curve, helios/src/x64/helios_map_to_curve.cpp:184-189).  Outputs the reference
