@@ -212,6 +212,102 @@ def run_pedersen(hs, torch, args, count=1024, m=4096):
     return res
 
 
+def run_serving(hs, torch, args):
+    """Served request stream: Poisson arrivals at a fixed offered load, sizes log-uniform in
+    [32, 2^18], curve by coin flip, seeded SSWU bases and uniform scalars resident in HBM (each
+    request reads its own slice of a per-curve pool).  The native server (continuous batching,
+    footprint admission) against plain per-request launches on the same arrival schedule, plus
+    both at saturation (all requests queued at once)."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed + 3000)
+    pool = 1 << 18
+    R, lam = args.serve_requests, args.serve_rate
+    sizes = np.exp(rng.uniform(np.log(32), np.log(pool), size=R)).astype(np.int64).clip(32, pool)
+    curves = rng.integers(0, 2, size=R)
+    offs = np.array([int(rng.integers(0, pool - n + 1)) for n in sizes])
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=R))
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        dsc, dpt = [], []
+        for c in (0, 1):
+            u = torch.from_numpy(rng.integers(0, 256, size=(pool, 32), dtype=np.uint8)).to(dev)
+            p = torch.empty((pool, 15), dtype=torch.int64, device=dev)
+            hs.map_to_curve_device(c, u.data_ptr(), pool, p.data_ptr(), None, st.cuda_stream)
+            s = rng.integers(0, 256, size=(pool, 32), dtype=np.uint8)
+            s[:, 31] &= 0x7F
+            dsc.append(torch.from_numpy(s).to(dev))
+            dpt.append(p)
+        torch.cuda.synchronize()
+
+    def req(i):
+        c, n, o = int(curves[i]), int(sizes[i]), int(offs[i])
+        return c, n, dsc[c].data_ptr() + 32 * o, dpt[c].data_ptr() + 120 * o
+
+    def served(schedule):
+        srv = hs.Server(device_bytes=args.serve_bytes << 30, lanes=args.serve_lanes)
+        out = [None] * R
+        for i in range(min(8, R)):  # warm the lanes
+            c, n, ds, dp = req(i)
+            srv.wait(srv.submit(c, d_scalars=ds, d_points=dp, n=n))
+        srv.reset_stats()
+        t0 = time.perf_counter()
+        tickets = []
+        for i in range(R):
+            if schedule:
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+            c, n, ds, dp = req(i)
+            tickets.append(srv.submit(c, d_scalars=ds, d_points=dp, n=n))
+        for i, t in enumerate(tickets):
+            out[i] = srv.wait(t)[0]
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+        srv.close()
+        return out, stats, wall
+
+    def plain(schedule):
+        sh = st.cuda_stream
+        o = torch.zeros(8, dtype=torch.int32, device=dev)
+        lat, out = [], [None] * R
+        t0 = time.perf_counter()
+        with torch.cuda.stream(st):
+            for i in range(R):
+                if schedule:
+                    dt = t0 + arrivals[i] - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                c, n, ds, dp = req(i)
+                hs.msm_device(c, ds, dp, n, o.data_ptr(), None, 0, sh)
+                st.synchronize()
+                done = time.perf_counter()
+                lat.append((done - (t0 + arrivals[i] if schedule else t0)) * 1e6)
+                out[i] = o.cpu().numpy().tobytes()
+        wall = time.perf_counter() - t0
+        lat = np.array(lat)
+        return out, {"p50_us": float(np.percentile(lat, 50)), "p99_us": float(np.percentile(lat, 99))}, wall
+
+    npts = int(sizes.sum())
+    s_out, s_st, s_wall = served(True)
+    p_out, p_st, p_wall = plain(True)
+    ss_out, ss_st, ss_wall = served(False)
+    ps_out, _, ps_wall = plain(False)
+    return {
+        "workload": f"{R} requests, Poisson arrivals at {lam:g}/s, sizes log-uniform in [32, 2^18], 50/50 curves, "
+                    "bases and scalars resident in HBM",
+        "requests": R, "points": npts, "offered_points_per_s": npts / float(arrivals[-1]),
+        "served": {"p50_ms": s_st["p50_us"] / 1e3, "p99_ms": s_st["p99_us"] / 1e3, "points_per_s": npts / s_wall,
+                   "batches": s_st["batches"], "lanes": args.serve_lanes},
+        "plain_per_request": {"p50_ms": p_st["p50_us"] / 1e3, "p99_ms": p_st["p99_us"] / 1e3,
+                              "points_per_s": npts / p_wall},
+        "saturated_points_per_s": {"served": npts / ss_wall, "plain": npts / ps_wall},
+        "saturated_speedup": ps_wall / ss_wall,
+        "bit_exact_served_vs_plain": s_out == p_out and ss_out == p_out and ps_out == p_out,
+    }
+
+
 def cpu_baseline(hs_pts_cpu, sc_cpu, n_sample):
     """The reference CPU MSM (x64 Pippenger behind its runtime dispatch) on the first n_sample
     inputs of each curve, one thread."""
@@ -254,6 +350,11 @@ def main():
     ap.add_argument("--no-mix", action="store_true")
     ap.add_argument("--pedersen-steps", type=int, default=3)
     ap.add_argument("--no-pedersen", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="offered load, requests per second")
+    ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--serve-bytes", type=int, default=16, help="server device budget, GiB")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams")
     args = ap.parse_args()
@@ -345,9 +446,18 @@ def main():
     }
     rec["batched_mix"] = None if args.no_mix else run_mix(hs, torch, args)
     rec["pedersen_batch"] = None if args.no_pedersen else run_pedersen(hs, torch, args)
+    rec["serving"] = None if args.no_serve else run_serving(hs, torch, args)
+    pts_cpu = [inputs[c][1].cpu().numpy().view(np.uint64) for c in (0, 1)]
+    sc_cpu = [inputs[c][0].cpu().numpy() for c in (0, 1)]
+    # the drop-in slot with host buffers: PCIe upload of 152 B per point included
+    t_host = 0.0
+    for c in (0, 1):
+        hs.msm(c, sc_cpu[c], pts_cpu[c])
+        t0 = time.perf_counter()
+        hs.msm(c, sc_cpu[c], pts_cpu[c])
+        t_host += time.perf_counter() - t0
+    rec["host_slot_points_per_s"] = 2 * n / t_host
     if not args.no_cpu:
-        pts_cpu = [inputs[c][1].cpu().numpy().view(np.uint64) for c in (0, 1)]
-        sc_cpu = [inputs[c][0].cpu().numpy() for c in (0, 1)]
         rec["cpu_baseline"] = cpu_baseline(pts_cpu, sc_cpu, min(n, args.cpu_sample))
     else:
         rec["cpu_baseline"] = None

@@ -211,53 +211,74 @@ template <class F> HS_HD void fe_neg(fe &r, const fe &a)
 
 template <class F> HS_HD void fe_dbl(fe &r, const fe &a) { fe_add<F>(r, a, a); }
 
-// 17 product columns (each < 2^62) -> normalised residue
+// 17 product columns (each < 9 * 2^58 < 2^61.2) -> normalised residue
 template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
 {
-    // full carry propagation: 29-bit limbs l[0..16] and l[17] = the rest (< 2^30)
-    uint32_t l[2 * NL];
-    uint64_t t = 0;
-#pragma unroll
-    for (int k = 0; k < 2 * NL - 1; k++)
-    {
-        t = c[k] + (t >> 29);
-        l[k] = (uint32_t)t & M29;
-    }
-    l[2 * NL - 1] = (uint32_t)(t >> 29);
     if (F::NR == 1)
     {
-        // w_k = l_k + R * l_{9+k}
-        uint64_t u = 0;
+        // Fold the high columns while they are still 64-bit sums: with h = lo + 2^32 hi,
+        //   h 2^(29 (9 + k)) = R lo 2^(29 k) + 8 R hi 2^(29 (k + 1))   (2^261 = R),
+        // adding < 2^43.5 to a column, so one carry pass over nine columns finishes the job.
+        uint64_t w[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+            w[k] = c[k];
+#pragma unroll
+        for (int k = 0; k < NL - 1; k++)
+        {
+            const uint64_t h = c[NL + k];
+            w[k] = mad64((uint32_t)h, F::r(0), w[k]);
+            w[k + 1] = mad64((uint32_t)(h >> 32), 8u * F::r(0), w[k + 1]);
+        }
+        uint64_t t = 0;
 #pragma unroll
         for (int k = 0; k < NL; k++)
         {
-            u = mad64(l[NL + k], F::r(0), (uint64_t)l[k] + (u >> 29));
-            r.v[k] = (uint32_t)u & M29;
+            t = w[k] + (t >> 29);
+            r.v[k] = (uint32_t)t & M29;
         }
-        fold_top<F>(r, (uint32_t)(u >> 29));
+        // t >> 29 < 2^33.2 times 2^261 = R
+        uint64_t u = (t >> 29) * (uint64_t)F::r(0) + r.v[0];
+        r.v[0] = (uint32_t)u & M29;
+        u = (u >> 29) + r.v[1];
+        r.v[1] = (uint32_t)u & M29;
+        r.v[2] += (uint32_t)(u >> 29); // 0 or 1: r.v[2] <= 2^29
     }
     else
     {
-        // first fold: 9 high limbs x 5 limbs of R into columns 0..12
-        uint64_t w[NL + F::NR - 1];
+        // High columns 9..16 as one number H = sum c[9+k] 2^(29 k), carried into digits h[0..9];
+        // the low columns stay 64-bit sums.  value = L + R H (2^261 = R, five limbs).
+        uint32_t h[NL + 1];
+        uint64_t t = 0;
 #pragma unroll
-        for (int k = 0; k < NL + F::NR - 1; k++)
-            w[k] = k < NL ? l[k] : 0;
+        for (int k = 0; k < NL - 1; k++)
+        {
+            t = c[NL + k] + (t >> 29);
+            h[k] = (uint32_t)t & M29;
+        }
+        t >>= 29;
+        h[NL - 1] = (uint32_t)t & M29;
+        h[NL] = (uint32_t)(t >> 29);
+        // first fold into columns 0..13 (each < 2^61.2 + 5 * 2^58 < 2^62)
+        uint64_t w[NL + F::NR];
 #pragma unroll
-        for (int i = 0; i < NL; i++)
+        for (int k = 0; k < NL + F::NR; k++)
+            w[k] = k < NL ? c[k] : 0;
+#pragma unroll
+        for (int i = 0; i <= NL; i++)
 #pragma unroll
             for (int j = 0; j < F::NR; j++)
-                w[i + j] = mad64(l[NL + i], F::r(j), w[i + j]);
+                w[i + j] = mad64(h[i], F::r(j), w[i + j]);
+        // L + R H < 2^395, so the carried digits m[0..13] have m[13] < 2^19
         uint32_t m[NL + F::NR];
         uint64_t u = 0;
 #pragma unroll
-        for (int k = 0; k < NL + F::NR - 1; k++)
+        for (int k = 0; k < NL + F::NR; k++)
         {
             u = w[k] + (u >> 29);
             m[k] = (uint32_t)u & M29;
         }
-        m[NL + F::NR - 1] = (uint32_t)(u >> 29);
-        // second fold: limbs 9..13 (about 133 bits) x R into columns 0..8
+        // second fold: digits 9..13 (about 134 bits) x R into columns 0..8
         uint64_t x[NL];
 #pragma unroll
         for (int k = 0; k < NL; k++)
@@ -406,7 +427,7 @@ template <class F> HS_HD void fe_canon(fe &r, const fe &a)
     fe_from_words(r, w);
 }
 
-template <class F> HS_HD bool fe_is_zero(const fe &a)
+template <class F> HS_HD bool fe_is_zero_canon(const fe &a)
 {
     uint32_t w[8];
     fe_to_words_canon<F>(w, a);
@@ -415,6 +436,29 @@ template <class F> HS_HD bool fe_is_zero(const fe &a)
     for (int i = 0; i < 8; i++)
         o |= w[i];
     return o == 0;
+}
+
+// Zero test with a cheap exclusion first.  After one carry pass the value is D (strict 29-bit
+// digits d[0..8]) plus c * 2^261.  With c = 0, D < 2^261 and D = lo + 2^255 hi (hi < 2^6), so
+// D = 0 (mod m) needs lo + hi * (2^255 mod m) to be 0 or m (it is < 2m).  Both m = 2^255 - 19
+// and m = 2^255 - gamma (gamma < 2^127) have all bits 145..254 set, as does m - k (2^255 mod m)
+// for k <= 64, so digits 5..7 (bits 145..231) are then all zero or all ones.  Otherwise the
+// value is certainly nonzero; the full canonical test runs only in the rare remaining cases.
+template <class F> HS_HD bool fe_is_zero(const fe &a)
+{
+    uint32_t d[NL];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        const uint32_t t = a.v[i] + c;
+        d[i] = t & M29;
+        c = t >> 29;
+    }
+    const uint32_t o = d[5] | d[6] | d[7], n = d[5] & d[6] & d[7];
+    if (c == 0 && o != 0 && n != M29)
+        return false;
+    return fe_is_zero_canon<F>(a);
 }
 
 template <class F> HS_HD bool fe_eq(const fe &a, const fe &b)

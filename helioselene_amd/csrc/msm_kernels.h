@@ -433,6 +433,34 @@ __device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const uint32_
     xyzz_madd<F>(acc, acc_id, p);
 }
 
+// Raw 64-byte base of an entry (point 0 for padding, which add_entry_raw then skips); issued one
+// entry ahead in the flat kernel so the gather overlaps the current addition.
+__device__ __forceinline__ void load_entry_raw(uint4 (&q)[4], const uint32_t *__restrict__ pts, uint32_t e)
+{
+    const uint32_t idx = e == ENTRY_SKIP ? 0u : (e & 0x7fffffffu);
+    const uint4 *p = reinterpret_cast<const uint4 *>(pts + 16ull * idx);
+    q[0] = p[0];
+    q[1] = p[1];
+    q[2] = p[2];
+    q[3] = p[3];
+}
+
+template <class C>
+__device__ __forceinline__ void add_entry_raw(xyzz &acc, bool &acc_id, const uint4 (&q)[4], uint32_t e)
+{
+    using F = typename C::F;
+    if (e == ENTRY_SKIP)
+        return;
+    const uint32_t xw[8] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w};
+    const uint32_t yw[8] = {q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
+    aff p;
+    fe_from_words(p.x, xw);
+    fe_from_words(p.y, yw);
+    if (e >> 31)
+        fe_neg<F>(p.y, p.y);
+    xyzz_madd<F>(acc, acc_id, p);
+}
+
 // one thread per bucket with at most `lmax` entries; larger buckets are queued for k_bucket_big
 template <class C>
 __global__ void __launch_bounds__(256) k_bucket_small(const uint32_t *__restrict__ offsets, uint32_t nbuckets,
@@ -562,14 +590,24 @@ __global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict_
     bool acc_id = true;
     xyzz_identity<F>(acc);
     uint32_t pos = beg;
-    uint32_t e_next = sorted[pos];
+    uint32_t e_cur = sorted[pos];
+    uint4 q[4];
+    load_entry_raw(q, pts, e_cur);
+    uint32_t e_next = pos + 1 < end ? sorted[pos + 1] : ENTRY_SKIP;
     while (true)
     {
-        const uint32_t e = e_next;
+        const uint32_t e = e_cur;
+        const uint4 cq[4] = {q[0], q[1], q[2], q[3]};
         pos++;
         if (pos < end)
-            e_next = sorted[pos]; // issued before the arithmetic on e
-        add_entry<C>(acc, acc_id, pts, e);
+        {
+            // the next entry's base and the entry after it are in flight during this addition
+            e_cur = e_next;
+            load_entry_raw(q, pts, e_cur);
+            if (pos + 1 < end)
+                e_next = sorted[pos + 1];
+        }
+        add_entry_raw<C>(acc, acc_id, cq, e);
         const bool run_done = pos == bucket_end || pos == end;
         if (run_done)
         {

@@ -104,6 +104,34 @@ extern "C" void hst_fe_op_raw(int field, int o, uint8_t *out40, const uint8_t *p
     out40[39] = mx > (1u << 29) ? 1 : 0; // limb bound violated
 }
 
+// ops on raw limbs (any normalised limbs, up to 2^29 each): result value (40 bytes, byte 39 = limb
+// bound flag) and, for o = 4, the zero test
+extern "C" int hst_fe_op_limbs(int field, int o, uint8_t *out40, const uint32_t *la, const uint32_t *lb)
+{
+    fe a, b, r;
+    for (int i = 0; i < NL; i++)
+    {
+        a.v[i] = la[i];
+        b.v[i] = lb[i];
+    }
+    if (o == 4)
+        return field == 0 ? (int)fe_is_zero<Fp>(a) : (int)fe_is_zero<Fq>(a);
+    if (field == 0)
+    {
+        switch (o) { case 0: fe_mul<Fp>(r, a, b); break; case 1: fe_sq<Fp>(r, a); break; case 2: fe_add<Fp>(r, a, b); break; default: fe_sub<Fp>(r, a, b); }
+    }
+    else
+    {
+        switch (o) { case 0: fe_mul<Fq>(r, a, b); break; case 1: fe_sq<Fq>(r, a); break; case 2: fe_add<Fq>(r, a, b); break; default: fe_sub<Fq>(r, a, b); }
+    }
+    st_raw(out40, r);
+    uint32_t mx = 0;
+    for (int i = 0; i < NL; i++)
+        mx = r.v[i] > mx ? r.v[i] : mx;
+    out40[39] = mx > (1u << 29) ? 1 : 0;
+    return 0;
+}
+
 extern "C" void hst_fe_op(int field, int o, uint8_t *out, const uint8_t *a, const uint8_t *b)
 {
     g_field = field;

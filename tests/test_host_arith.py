@@ -167,6 +167,51 @@ def test_field_output_bounds(L, k):
             assert v < (1 << 262) and v % m == want % m, (o, hex(a), hex(b))
 
 
+def _limbs_value(l):
+    return sum(x << (29 * i) for i, x in enumerate(l))
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_field_extreme_limbs(L, k):
+    """Inputs at the representation's limits (every limb 2^29, or 2^29 - 1, mixed with zeros):
+    products and squares stay exact and normalised; the zero test is exact on multiples of m
+    written with non-canonical limbs."""
+    m = O.FIELD[k]
+    rnd = random.Random(31 + k)
+    top = 1 << 29
+    pats = [[top] * 9, [top - 1] * 9, [top, 0] * 4 + [top], [0] * 8 + [top], [top] + [0] * 8]
+    pats += [[rnd.choice([0, top, top - 1, rnd.getrandbits(29)]) for _ in range(9)] for _ in range(400)]
+    out = ctypes.create_string_buffer(40)
+    for i, la in enumerate(pats):
+        lb = pats[(i * 7 + 1) % len(pats)]
+        a, b = _limbs_value(la), _limbs_value(lb)
+        A, B = (ctypes.c_uint32 * 9)(*la), (ctypes.c_uint32 * 9)(*lb)
+        for o, want in ((0, a * b), (1, a * a), (2, a + b), (3, a - b)):
+            L.hst_fe_op_limbs(k, o, out, A, B)
+            assert out.raw[39] == 0, ("limb bound", o, la, lb)
+            v = int.from_bytes(out.raw[:39], "little")
+            assert v < (1 << 262) and v % m == want % m, (o, la, lb)
+    # zero test: j*m for j = 0..64 in strict digits, and with one digit borrowed into the next
+    for j in list(range(65)):
+        v = j * m
+        d = [(v >> (29 * i)) & (top - 1) for i in range(9)]
+        assert L.hst_fe_op_limbs(k, 4, out, (ctypes.c_uint32 * 9)(*d), (ctypes.c_uint32 * 9)()) == 1, j
+        for pos in range(8):
+            if d[pos + 1] > 0:
+                e = list(d)
+                e[pos + 1] -= 1
+                e[pos] += top  # same value, limb pos = digit + 2^29 (<= 2^30)
+                if e[pos] <= top:
+                    assert L.hst_fe_op_limbs(k, 4, out, (ctypes.c_uint32 * 9)(*e), (ctypes.c_uint32 * 9)()) == 1
+    for _ in range(2000):
+        la = [rnd.getrandbits(29) for _ in range(9)]
+        want = _limbs_value(la) % m == 0
+        assert L.hst_fe_op_limbs(k, 4, out, (ctypes.c_uint32 * 9)(*la), (ctypes.c_uint32 * 9)()) == int(want)
+    for v in (1, m - 1, m + 1, 2 * m - 1, (1 << 255), 19, 1 << 145):
+        d = [(v >> (29 * i)) & (top - 1) for i in range(9)]
+        assert L.hst_fe_op_limbs(k, 4, out, (ctypes.c_uint32 * 9)(*d), (ctypes.c_uint32 * 9)()) == 0, hex(v)
+
+
 @pytest.mark.parametrize("c", [0, 1])
 def test_jacobian_chain_vs_oracle(L, c):
     """xyzz -> Jacobian conversion, dbl-2001-b chains and add-2007-bl (including P + P and
