@@ -8,8 +8,8 @@
 // so all MSMs over one base set share the 5 doublings per window (255 in total).
 //
 // GPU mapping: one wavefront per (MSM k, window w) accumulates S_kw over all bases (lane l takes
-// bases l, l+64, ...) with mixed additions, then a 64-way LDS tree sum.  One thread per MSM runs
-// the Horner chain over its 52 window sums.  The digit of window w is read locally from the
+// bases l, l+64, ...) with mixed additions, then a 64-way LDS tree sum.  One quad of lanes per MSM
+// runs the Horner chain over its 52 window sums.  The digit of window w is read locally from the
 // canonical scalar: d = bits[5w, 5w+5) + bit(5w-1) - 32 * bit(5w+4), in [-16, 16]; this signed
 // recoding differs from helios_scalar_recode_signed5 digit by digit but represents the same
 // integer, and group arithmetic here is exact, so every result is the same point.
@@ -189,29 +189,31 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
     }
 }
 
-// One thread per MSM: Horner over the 52 window sums (5 shared doublings per window), then the
-// 32-byte encoding and the reference struct (affine, Z = 1; identity 1:1:0).
+// One quad of lanes per MSM: Horner over the 52 window sums (5 shared doublings per window),
+// then the 32-byte encoding and the reference struct (affine, Z = 1; identity 1:1:0).
 template <class C>
 __global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S, uint32_t K,
                                                      uint32_t *__restrict__ out32, uint64_t *__restrict__ outjac)
 {
     using F = typename C::F;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (k >= K)
         return;
     const xyzz *sk = S + (size_t)k * FIXED_WINDOWS;
     xyzz s;
     jac acc, t;
     load_xyzz(s, sk + FIXED_WINDOWS - 1);
-    xyzz_to_jac<F>(acc, s);
+    xyzz_to_jac_q<F>(acc, s);
     for (int w = FIXED_WINDOWS - 2; w >= 0; w--)
     {
         for (int j = 0; j < 5; j++)
-            jac_dbl<F>(acc, acc);
+            jac_dbl_q<F>(acc);
         load_xyzz(s, sk + w);
-        xyzz_to_jac<F>(t, s);
-        jac_add<F>(acc, acc, t);
+        xyzz_to_jac_q<F>(t, s);
+        jac_add_q<F>(acc, acc, t);
     }
+    if ((threadIdx.x & 3) != 0)
+        return;
     aff a;
     const bool pt = jac_to_aff<F>(a, acc);
     uint32_t enc[8];

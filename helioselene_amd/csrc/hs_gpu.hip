@@ -435,7 +435,7 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
     hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((g.nseg + 255) / 256), blk, 0, st, buckets, offsets, g.B, g.G, 0u, g.nseg,
                        seg);
     hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(g.V), dim3(hs::BIG_THREADS), 0, st, seg, g.B / g.G, 0u, win);
-    hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
+    hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((4 * g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
                        g.total.as<hs::jac>());
     hipLaunchKernelGGL(hs::k_finalize_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, g.total.as<hs::jac>(), d, g.M);
     HS_CHECK(hipGetLastError());
@@ -796,7 +796,7 @@ int fixed_comb_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, uns
                        (const uint32_t *)d_scalars, rows, f.scal.as<uint32_t>(), compat);
     hipLaunchKernelGGL(hs::k_fixed_window_sums<C>, dim3((unsigned)(count * hs::FIXED_WINDOWS)), dim3(64), 0, st,
                        f.tables.as<uint32_t>(), f.scal.as<uint32_t>(), f.n, f.sums.as<hs::xyzz>());
-    hipLaunchKernelGGL(hs::k_fixed_horner<C>, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st,
+    hipLaunchKernelGGL(hs::k_fixed_horner<C>, dim3((unsigned)((4 * count + 63) / 64)), dim3(64), 0, st,
                        f.sums.as<hs::xyzz>(), (uint32_t)count, (uint32_t *)d_out32, (uint64_t *)d_outjac);
     HS_CHECK(hipGetLastError());
     return 0;

@@ -13,6 +13,7 @@
 
 #include "curve.h"
 #include "scalar.h"
+#include "quad.h"
 
 namespace hs
 {
@@ -785,12 +786,13 @@ __device__ __forceinline__ void force_vgpr(xyzz &p)
 
 // Horner step over windows [w_lo, w_hi), highest first: acc = 2^c acc + S_w.  `first` starts
 // from the identity, otherwise from the Jacobian value in *total; the result goes back to *total.
+// One quad of lanes runs the chain (quad-parallel doublings and additions).
 template <class C>
 __global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out, int w_lo, int w_hi, int c, int first,
                                                jac *total)
 {
     using F = typename C::F;
-    if (threadIdx.x != 0 || blockIdx.x != 0)
+    if (threadIdx.x >= 4 || blockIdx.x != 0)
         return;
     xyzz s;
     jac acc, t;
@@ -798,25 +800,21 @@ __global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out,
     if (first)
     {
         load_xyzz(s, win_out + w);
-        force_vgpr(s);
-        xyzz_to_jac<F>(acc, s);
+        xyzz_to_jac_q<F>(acc, s);
         w--;
     }
     else
-    {
         acc = *total;
-        force_vgpr_jac(acc);
-    }
     for (; w >= w_lo; w--)
     {
         for (int k = 0; k < c; k++)
-            jac_dbl<F>(acc, acc);
+            jac_dbl_q<F>(acc);
         load_xyzz(s, win_out + w);
-        force_vgpr(s);
-        xyzz_to_jac<F>(t, s);
-        jac_add<F>(acc, acc, t);
+        xyzz_to_jac_q<F>(t, s);
+        jac_add_q<F>(acc, acc, t);
     }
-    *total = acc;
+    if (threadIdx.x == 0)
+        *total = acc;
 }
 
 // out32: compressed point (8 words); out_jac: reference Jacobian struct (15 uint64): (x : y : 1)
@@ -1171,29 +1169,30 @@ __global__ void __launch_bounds__(256) k_window_sort(const BatchDesc *__restrict
         sorted[j] = ENTRY_SKIP;
 }
 
-// Horner per MSM (one thread each) over its W window sums, then the affine encoding.
+// Horner per MSM (one quad of lanes each) over its W window sums.
 template <class C>
 __global__ void __launch_bounds__(64) k_horner_batch(const xyzz *__restrict__ win_out, uint32_t M, int W, int c,
                                                      jac *__restrict__ total)
 {
     using F = typename C::F;
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (m >= M)
         return;
     const xyzz *ws = win_out + (size_t)m * W;
     xyzz s;
     jac acc, t;
     load_xyzz(s, ws + (W - 1));
-    xyzz_to_jac<F>(acc, s);
+    xyzz_to_jac_q<F>(acc, s);
     for (int w = W - 2; w >= 0; w--)
     {
         for (int k = 0; k < c; k++)
-            jac_dbl<F>(acc, acc);
+            jac_dbl_q<F>(acc);
         load_xyzz(s, ws + w);
-        xyzz_to_jac<F>(t, s);
-        jac_add<F>(acc, acc, t);
+        xyzz_to_jac_q<F>(t, s);
+        jac_add_q<F>(acc, acc, t);
     }
-    total[m] = acc;
+    if ((threadIdx.x & 3) == 0)
+        total[m] = acc;
 }
 
 template <class C>

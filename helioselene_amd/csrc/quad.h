@@ -1,0 +1,158 @@
+// Point arithmetic spread over a quad of lanes, for the latency-bound serial chains (Horner
+// combines, window-sum conversions).
+//
+// A single lane walking 256 doublings is issue-bound: every field product is ~200 dependent-ish
+// VALU instructions and no other wave hides them.  Here the four lanes of a quad hold the same
+// point; each lane computes one of up to four independent products of a formula level, and a DPP
+// quad_perm broadcast (one VALU instruction per limb) hands every result to the whole quad.
+// dbl-2001-b then takes 4 product levels instead of 8 products, add-2007-bl 5 instead of 16.
+// All four lanes of a quad must be active; results are identical in the four lanes.
+#pragma once
+#include "curve.h"
+
+namespace hs
+{
+
+template <int K> __device__ __forceinline__ void fe_qbcast(fe &r, const fe &a)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)a.v[i], K * 0x55, 0xf, 0xf, false);
+}
+
+// lane q of the quad takes a_q (masks keep the operands in registers: a select between the
+// structs themselves would be lowered to a select of their stack addresses)
+__device__ __forceinline__ void fe_qsel(fe &r, const fe &a0, const fe &a1, const fe &a2, const fe &a3)
+{
+    const uint32_t q = threadIdx.x & 3u;
+    const uint32_t m0 = 0u - (uint32_t)(q == 0), m1 = 0u - (uint32_t)(q == 1), m2 = 0u - (uint32_t)(q == 2),
+                   m3 = 0u - (uint32_t)(q == 3);
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = (a0.v[i] & m0) | (a1.v[i] & m1) | (a2.v[i] & m2) | (a3.v[i] & m3);
+}
+
+// four independent products a_k * b_k, one per lane of the quad
+template <class F>
+__device__ __forceinline__ void fe_mul_q4(fe &r0, fe &r1, fe &r2, fe &r3, const fe &a0, const fe &b0, const fe &a1,
+                                          const fe &b1, const fe &a2, const fe &b2, const fe &a3, const fe &b3)
+{
+    fe a, b, r;
+    fe_qsel(a, a0, a1, a2, a3);
+    fe_qsel(b, b0, b1, b2, b3);
+    fe_mul<F>(r, a, b);
+    fe_qbcast<0>(r0, r);
+    fe_qbcast<1>(r1, r);
+    fe_qbcast<2>(r2, r);
+    fe_qbcast<3>(r3, r);
+}
+
+template <class F>
+__device__ __forceinline__ void fe_mul_q3(fe &r0, fe &r1, fe &r2, const fe &a0, const fe &b0, const fe &a1,
+                                          const fe &b1, const fe &a2, const fe &b2)
+{
+    fe a, b, r;
+    fe_qsel(a, a0, a1, a2, a2);
+    fe_qsel(b, b0, b1, b2, b2);
+    fe_mul<F>(r, a, b);
+    fe_qbcast<0>(r0, r);
+    fe_qbcast<1>(r1, r);
+    fe_qbcast<2>(r2, r);
+}
+
+template <class F>
+__device__ __forceinline__ void fe_mul_q2(fe &r0, fe &r1, const fe &a0, const fe &b0, const fe &a1, const fe &b1)
+{
+    fe a, b, r;
+    fe_qsel(a, a0, a1, a1, a1);
+    fe_qsel(b, b0, b1, b1, b1);
+    fe_mul<F>(r, a, b);
+    fe_qbcast<0>(r0, r);
+    fe_qbcast<1>(r1, r);
+}
+
+// dbl-2001-b (a = -3), same formula as jac_dbl: levels {Z^2, Y^2, (Y+Z)^2}, {X gamma,
+// (X - delta)(X + delta), gamma^2}, {alpha^2}, {alpha (4 beta - X3)}.
+template <class F> __device__ __forceinline__ void jac_dbl_q(jac &p)
+{
+    fe yz, delta, gamma, yz2, xm, xp, beta, t, g2, alpha, X3, Y3, Z3, t0;
+    fe_add<F>(yz, p.Y, p.Z);
+    fe_mul_q3<F>(delta, gamma, yz2, p.Z, p.Z, p.Y, p.Y, yz, yz);
+    fe_sub<F>(xm, p.X, delta);
+    fe_add<F>(xp, p.X, delta);
+    fe_mul_q3<F>(beta, t, g2, p.X, gamma, xm, xp, gamma, gamma);
+    fe_mul_small<F>(alpha, t, 3);
+    fe_sq<F>(X3, alpha);
+    fe_mul_small<F>(t0, beta, 8);
+    fe_sub<F>(X3, X3, t0);
+    fe_sub<F>(Z3, yz2, gamma);
+    fe_sub<F>(Z3, Z3, delta);
+    fe_mul_small<F>(t0, beta, 4);
+    fe_sub<F>(t0, t0, X3);
+    fe_mul<F>(Y3, alpha, t0);
+    fe_mul_small<F>(t0, g2, 8);
+    fe_sub<F>(Y3, Y3, t0);
+    p.X = X3;
+    p.Y = Y3;
+    p.Z = Z3;
+}
+
+// xyzz -> Jacobian (Z' = ZZ ZZZ, X' = X ZZ ZZZ^2, Y' = (Y ZZ) Z'^2), three levels
+template <class F> __device__ __forceinline__ void xyzz_to_jac_q(jac &r, const xyzz &p)
+{
+    fe Z, zzz2, t, u, z2;
+    fe_mul_q4<F>(Z, zzz2, t, u, p.ZZ, p.ZZZ, p.ZZZ, p.ZZZ, p.X, p.ZZ, p.Y, p.ZZ);
+    fe_mul_q2<F>(r.X, z2, t, zzz2, Z, Z);
+    fe_mul<F>(r.Y, u, z2);
+    r.Z = Z;
+}
+
+// add-2007-bl with the identity / equal / opposite cases, same results as jac_add
+template <class F> __device__ __forceinline__ void jac_add_q(jac &r, const jac &p, const jac &q)
+{
+    if (fe_is_zero<F>(p.Z))
+    {
+        r = q;
+        return;
+    }
+    if (fe_is_zero<F>(q.Z))
+    {
+        r = p;
+        return;
+    }
+    fe Z1Z1, Z2Z2, zs, zs2, U1, U2, a, b, S1, S2, H, rr, I, Zt, Z3, J, V, r2, X3, Y3, t0, t1;
+    fe_add<F>(zs, p.Z, q.Z);
+    fe_mul_q3<F>(Z1Z1, Z2Z2, zs2, p.Z, p.Z, q.Z, q.Z, zs, zs);
+    fe_mul_q4<F>(U1, U2, a, b, p.X, Z2Z2, q.X, Z1Z1, q.Z, Z2Z2, p.Z, Z1Z1);
+    fe_sub<F>(H, U2, U1);
+    fe_add<F>(t0, H, H);
+    fe_sub<F>(Zt, zs2, Z1Z1);
+    fe_sub<F>(Zt, Zt, Z2Z2);
+    fe_mul_q4<F>(S1, S2, I, Z3, p.Y, a, q.Y, b, t0, t0, Zt, H);
+    fe_sub<F>(rr, S2, S1);
+    if (fe_is_zero<F>(H))
+    {
+        if (fe_is_zero<F>(rr))
+        {
+            r = p;
+            jac_dbl_q<F>(r);
+        }
+        else
+            jac_identity<F>(r);
+        return;
+    }
+    fe_add<F>(rr, rr, rr);
+    fe_mul_q3<F>(J, V, r2, H, I, U1, I, rr, rr);
+    fe_sub<F>(X3, r2, J);
+    fe_add<F>(t0, V, V);
+    fe_sub<F>(X3, X3, t0);
+    fe_sub<F>(t0, V, X3);
+    fe_mul_q2<F>(t1, Y3, rr, t0, S1, J);
+    fe_add<F>(t0, Y3, Y3);
+    fe_sub<F>(Y3, t1, t0);
+    r.X = X3;
+    r.Y = Y3;
+    r.Z = Z3;
+}
+
+} // namespace hs
