@@ -270,8 +270,9 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     // two-level sort geometry: nsb super-buckets of 2^shift buckets per window, blocks of T points
     const int shift = P.c - 1 > 8 ? P.c - 1 - 8 : 0;
     const int nsb = (int)(P.B >> shift);
-    uint32_t T = 4096;
-    while ((nn + T - 1) / T > 512)
+    // points per coarse block: about 256 blocks (one per CU), at least 256 points each
+    uint32_t T = 256;
+    while ((nn + T - 1) / T > 256)
         T *= 2;
     const uint32_t nblk = (nn + T - 1) / T;
     const uint32_t nh = (uint32_t)P.W * nsb;

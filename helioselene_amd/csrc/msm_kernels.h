@@ -329,13 +329,16 @@ __global__ void __launch_bounds__(256) k_coarse_count(const uint16_t *__restrict
     for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
         hist[j] = 0;
     __syncthreads();
-    const uint32_t tot = (uint32_t)W * cnt;
-    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x)
+    for (int w = 0; w < W; w++)
     {
-        const uint32_t w = idx / cnt, i = beg + idx % cnt;
-        const uint16_t code = digits[(size_t)w * n + i];
-        if (code != DIGIT_ZERO)
-            atomicAdd(&hist[w * nsb + ((code & 0x7fffu) >> shift)], 1u);
+        const uint16_t *dw = digits + (size_t)w * n + beg;
+        uint32_t *hw = hist + (uint32_t)w * nsb;
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x)
+        {
+            const uint16_t code = dw[i];
+            if (code != DIGIT_ZERO)
+                atomicAdd(&hw[(code & 0x7fffu) >> shift], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
@@ -353,16 +356,20 @@ __global__ void __launch_bounds__(256) k_coarse_scatter(const uint16_t *__restri
     for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
         cur[j] = offs[(size_t)j * nblk + b];
     __syncthreads();
-    const uint32_t tot = (uint32_t)W * cnt, smask = (1u << shift) - 1;
-    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x)
+    const uint32_t smask = (1u << shift) - 1;
+    for (int w = 0; w < W; w++)
     {
-        const uint32_t w = idx / cnt, i = beg + idx % cnt;
-        const uint16_t code = digits[(size_t)w * n + i];
-        if (code != DIGIT_ZERO)
+        const uint16_t *dw = digits + (size_t)w * n;
+        uint32_t *cw = cur + (uint32_t)w * nsb;
+        for (uint32_t i = beg + threadIdx.x; i < beg + cnt; i += blockDim.x)
         {
-            const uint32_t mag1 = code & 0x7fffu;
-            const uint32_t pos = atomicAdd(&cur[w * nsb + (mag1 >> shift)], 1u);
-            entries[pos] = i | ((mag1 & smask) << 24) | ((uint32_t)(code >> 15) << 31);
+            const uint16_t code = dw[i];
+            if (code != DIGIT_ZERO)
+            {
+                const uint32_t mag1 = code & 0x7fffu;
+                const uint32_t pos = atomicAdd(&cw[mag1 >> shift], 1u);
+                entries[pos] = i | ((mag1 & smask) << 24) | ((uint32_t)(code >> 15) << 31);
+            }
         }
     }
 }
