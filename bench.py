@@ -44,6 +44,14 @@ def algorithmic_bucket_ops(curve, n):
     return 2 * W * n * (5 * m + s), c
 
 
+def msm_int32_ops(curve, n):
+    """INT32 multiply ops of a whole n-point MSM under the fixed convention: min over c of
+    W(c) n (5m+s) + W(c) 2^c (11m+5s), plus 256 doublings at 3m+5s, 2 ops per product."""
+    m, s = PRODUCTS[curve]
+    best = min(-(-256 // c) * n * (5 * m + s) + -(-256 // c) * (1 << c) * (11 * m + 5 * s) for c in range(4, 21))
+    return 2 * (best + 256 * (3 * m + 5 * s))
+
+
 def make_inputs(hs, torch, curve, n, seed, dev):
     import numpy as np
 
@@ -355,6 +363,7 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-bytes", type=int, default=16, help="server device budget, GiB")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--roofline-steps", type=int, default=3)
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams")
     args = ap.parse_args()
@@ -386,13 +395,25 @@ def main():
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
-        hs.profile(True)
-        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        # HBM counter calibration: one plain 256 MiB device copy (read 256 MiB, write 256 MiB), so a
+        # FETCH_SIZE / WRITE_SIZE counter run of this command can scale the MSM kernels' counts
+        cal = torch.empty(1 << 26, dtype=torch.int32, device=dev)
+        cal2 = torch.empty_like(cal)
+        cal2.copy_(cal)
+        torch.cuda.synchronize()
+        del cal, cal2
+        # roofline pass: the same MSMs one at a time on one stream, HIP events between phases
+        hs.profile(True)
+        for _ in range(args.roofline_steps):
+            for c in (0, 1):
+                sc, pts = inputs[c]
+                hs.msm_device(c, sc.data_ptr(), pts.data_ptr(), n, out[c].data_ptr(), None, 0, sts[0])
+                torch.cuda.synchronize()
         phases = [hs.phase_summary(c) for c in (0, 1)]
         hs.profile(False)
 
@@ -405,6 +426,8 @@ def main():
     peak = 2 * peak_products
     per_curve_ms = {["helios", "selene"][c]: {k: v / max(1, phases[c][1]) for k, v in phases[c][0].items()}
                     for c in (0, 1)}
+    msm_ms = [sum(per_curve_ms[cv].values()) for cv in ("helios", "selene")]
+    msm_ops = [msm_int32_ops(c, n) for c in (0, 1)]
 
     rec = {
         "metric": METRIC,
@@ -430,7 +453,7 @@ def main():
         },
         "roofline": {
             "bound": "valu_int32",
-            "kernel": "k_bucket (bucket accumulation)",
+            "kernel": "k_bucket_flat (bucket accumulation)",
             "achieved": achieved / 1e12,
             "peak": peak / 1e12,
             "unit": "TOP/s",
@@ -438,11 +461,15 @@ def main():
             "traffic": None,
             "algorithmic_ops_per_launch": {"helios": ops[0], "selene": ops[1], "c_star": list(cstar)},
             "kernel_ms": {"helios": bucket_ms[0], "selene": bucket_ms[1]},
+            "kernel_ms_source": "HIP events around k_chunk_start + k_bucket_flat on the MSM's stream, "
+                                f"{args.roofline_steps} isolated MSMs per curve after the timed steps",
             "peak_source": "measured v_mad_u64_u32 throughput x 2 INT32 ops per partial product",
+            "whole_msm": {cv: {"ms": msm_ms[c], "int32_ops": msm_ops[c],
+                               "frac": msm_ops[c] / (msm_ms[c] * 1e-3) / peak}
+                          for c, cv in enumerate(("helios", "selene"))},
         },
-        "phase_ms": per_curve_ms,
-        "per_curve_points_per_s": {["helios", "selene"][c]: n / (sum(per_curve_ms[["helios", "selene"][c]].values()) * 1e-3)
-                                   for c in (0, 1)},
+        "phase_ms_isolated": per_curve_ms,
+        "per_curve_points_per_s_isolated": {cv: n / (msm_ms[c] * 1e-3) for c, cv in enumerate(("helios", "selene"))},
     }
     rec["batched_mix"] = None if args.no_mix else run_mix(hs, torch, args)
     rec["pedersen_batch"] = None if args.no_pedersen else run_pedersen(hs, torch, args)

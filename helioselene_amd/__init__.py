@@ -91,6 +91,10 @@ def lib():
         L.hs_gpu_server_get_stats.argtypes = [vp, vp]
         L.hs_gpu_server_reset_stats.argtypes = [vp]
         L.hs_gpu_server_destroy.argtypes = [vp]
+        L.hs_gpu_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, cp]
+        L.hs_gpu_tune_load.argtypes = [cp]
+        L.hs_gpu_tune_get.argtypes = [ctypes.c_int, ctypes.c_int, sz, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.hs_gpu_debug_dump.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
         _lib = L
     return _lib
@@ -489,3 +493,24 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- autotuner
+def autotune(curve, mode, lg_lo, lg_hi, path=None):
+    """Measure and install the fastest (c, L, G) per size class 2^lg_lo..2^lg_hi (mode 0 plain,
+    1 batch groups); optionally append them to a tuning file."""
+    _check(lib().hs_gpu_autotune(CURVES[curve], mode, lg_lo, lg_hi, path.encode() if path else None))
+
+
+def tune_load(path):
+    _check(lib().hs_gpu_tune_load(path.encode()))
+
+
+def tune_get(curve, mode, n):
+    c, L, G = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib().hs_gpu_tune_get(CURVES[curve], mode, n, ctypes.byref(c), ctypes.byref(L), ctypes.byref(G)))
+    return c.value, L.value, G.value
+
+
+def tune_clear():
+    _check(lib().hs_gpu_tune_clear())

@@ -269,34 +269,56 @@ template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
 #pragma unroll
             for (int j = 0; j < F::NR; j++)
                 w[i + j] = mad64(h[i], F::r(j), w[i + j]);
-        // L + R H < 2^395, so the carried digits m[0..13] have m[13] < 2^19
-        uint32_t m[NL + F::NR];
-        uint64_t u = 0;
+        // columns 9..13 again as one number M (< 2^134 since L + R H < 2^395), carried into
+        // digits m[0..4]; the second fold m_i R_j (i + j <= 8) lands in columns 0..8
+        uint32_t m[F::NR];
+        t = 0;
 #pragma unroll
-        for (int k = 0; k < NL + F::NR; k++)
+        for (int k = 0; k < F::NR; k++)
         {
-            u = w[k] + (u >> 29);
-            m[k] = (uint32_t)u & M29;
+            t = w[NL + k] + (t >> 29);
+            m[k] = (uint32_t)t & M29;
         }
-        // second fold: digits 9..13 (about 134 bits) x R into columns 0..8
-        uint64_t x[NL];
-#pragma unroll
-        for (int k = 0; k < NL; k++)
-            x[k] = m[k];
 #pragma unroll
         for (int i = 0; i < F::NR; i++)
 #pragma unroll
             for (int j = 0; j < F::NR; j++)
                 if (i + j < NL)
-                    x[i + j] = mad64(m[NL + i], F::r(j), x[i + j]);
-        u = 0;
+                    w[i + j] = mad64(m[i], F::r(j), w[i + j]);
+        // columns 0..8 (< 2^62.3) -> digits; the carry out of the top (< 2^34) times R
+        t = 0;
 #pragma unroll
         for (int k = 0; k < NL; k++)
         {
-            u = x[k] + (u >> 29);
+            t = w[k] + (t >> 29);
+            r.v[k] = (uint32_t)t & M29;
+        }
+        const uint64_t top = t >> 29;
+        const uint32_t tlo = (uint32_t)top & M29, thi = (uint32_t)(top >> 29);
+        // top * R = tlo * R + thi * R * 2^29: at most 6 columns, carried once more
+        uint64_t u = 0;
+#pragma unroll
+        for (int k = 0; k < F::NR + 1; k++)
+        {
+            uint64_t x = (uint64_t)r.v[k] + (u >> 29);
+            if (k < F::NR)
+                x = mad64(tlo, F::r(k), x);
+            if (k >= 1)
+                x = mad64(thi, F::r(k - 1), x);
+            u = x;
             r.v[k] = (uint32_t)u & M29;
         }
-        fold_top<F>(r, (uint32_t)(u >> 29));
+        // remaining carry (< 2^5) ripples into limb 6 at most once past a full limb
+        uint32_t cc = (uint32_t)(u >> 29);
+#pragma unroll
+        for (int k = F::NR + 1; k < NL; k++)
+        {
+            const uint32_t y = r.v[k] + cc;
+            r.v[k] = y & M29;
+            cc = y >> 29;
+        }
+        // cc (0 or 1) is 2^261 = R once more: limbs here are small enough to absorb it directly
+        fold_top<F>(r, cc);
     }
 }
 

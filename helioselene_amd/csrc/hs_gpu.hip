@@ -9,6 +9,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -92,20 +93,99 @@ int env_int(const char *name, int dflt)
     return v ? std::atoi(v) : dflt;
 }
 
-Plan make_plan(size_t n)
+// ---------------------------------------------------------------- tuned plan parameters
+// The autotuner's table: for (mode, curve, size class) the fastest (c, L, G) it measured.  mode 0 =
+// plain MSM, 1 = a group of a batch; size class = floor(log2 n).  Lookups fall back to the nearest
+// tuned class of the same mode and curve, then to the built-in defaults.
+struct TuneEntry
+{
+    int c = 0, L = 32, G = 8;
+    float ms = 0.f;
+};
+
+struct TuneTable
+{
+    std::mutex mu;
+    std::map<int, TuneEntry> e; // key = mode * 1024 + curve * 64 + lg
+    bool loaded_env = false;
+};
+
+TuneTable &tune()
+{
+    static TuneTable t;
+    return t;
+}
+
+int size_class(size_t n)
+{
+    int lg = 0;
+    while (((size_t)1 << (lg + 1)) <= n)
+        lg++;
+    return lg;
+}
+
+int tune_load_file(const char *path);
+
+// Forced parameters while the autotuner measures a candidate (set on the tuning thread only).
+thread_local const TuneEntry *g_force = nullptr;
+
+bool tune_lookup(int mode, int curve, size_t n, TuneEntry &out)
+{
+    if (g_force)
+    {
+        out = *g_force;
+        return true;
+    }
+    TuneTable &t = tune();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (!t.loaded_env)
+    {
+        t.loaded_env = true;
+        if (const char *f = std::getenv("HS_GPU_TUNE_FILE"))
+        {
+            t.mu.unlock();
+            tune_load_file(f);
+            t.mu.lock();
+        }
+    }
+    const int lg = size_class(n);
+    int best = -1, bd = 1 << 30;
+    for (const auto &kv : t.e)
+    {
+        const int m = kv.first / 1024, c = (kv.first / 64) % 16, l = kv.first % 64;
+        if (m != mode || c != curve)
+            continue;
+        const int d = l > lg ? l - lg : lg - l;
+        if (d < bd && d <= 1)
+        {
+            bd = d;
+            best = kv.first;
+        }
+    }
+    if (best < 0)
+        return false;
+    out = t.e[best];
+    return true;
+}
+
+Plan make_plan(int curve, size_t n)
 {
     Plan p;
-    p.c = env_int("HS_GPU_C", choose_c(n));
+    TuneEntry te;
+    const bool tuned = tune_lookup(0, curve, n, te);
+    p.c = env_int("HS_GPU_C", tuned ? te.c : choose_c(n));
     if (p.c < 4)
         p.c = 4;
     if (p.c > 16)
         p.c = 16;
     p.W = (256 + p.c - 1) / p.c;
     p.B = 1u << (p.c - 1);
-    uint32_t g = (uint32_t)env_int("HS_GPU_G", 8);
+    uint32_t g = (uint32_t)env_int("HS_GPU_G", tuned ? te.G : 8);
     p.G = p.B < g ? p.B : g;
     p.nb = (uint32_t)p.W * p.B;
-    p.L = (uint32_t)env_int("HS_GPU_L", 32);
+    p.L = (uint32_t)env_int("HS_GPU_L", tuned ? te.L : 32);
+    if (p.L < 1)
+        p.L = 1;
     p.Q = env_int("HS_GPU_Q", 1);
     if (p.Q < 1)
         p.Q = 1;
@@ -264,7 +344,7 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     if (n >= (1ull << 24))
         return fail(HS_GPU_EINVAL, "n must be below 2^24 points per MSM");
     Workspace &ws = cx.ws(st);
-    const Plan P = make_plan(n);
+    const Plan P = make_plan(C::id, n);
     const uint32_t nn = (uint32_t)n;
     const bool compat = !(flags & HS_GPU_EXACT) && ref_window(n) == 8;
     // two-level sort geometry: nsb super-buckets of 2^shift buckets per window, blocks of T points
@@ -309,7 +389,7 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     hipLaunchKernelGGL(hs::k_recode16<C>, grd, blk, 0, st, d_scalars, isid, nn, P.c, P.W, compat ? 1 : 0, digits);
     cx.mark(st, 1);
     hipLaunchKernelGGL(hs::k_coarse_count, dim3(nblk), blk, lds, st, digits, nn, P.W, nsb, shift, T, nblk, cnt_a);
-    HS_CHECK(hipMemsetAsync(cnt_a + (size_t)nh * nblk, 0, 4, st));
+    hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, cnt_a + (size_t)nh * nblk, 1u, 0u);
     if ((r = scan_exclusive(ws, cnt_a, off_a, nh * nblk + 1, st)))
         return r;
     hipLaunchKernelGGL(hs::k_coarse_scatter, dim3(nblk), blk, lds, st, digits, nn, P.W, nsb, shift, T, nblk, off_a,
@@ -330,15 +410,16 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         const uint32_t g_lo = (uint32_t)w_lo * P.B, g_hi = (uint32_t)w_hi * P.B, ng = g_hi - g_lo;
         const dim3 gnb((ng + 255) / 256);
         const size_t chunks = ((size_t)n * (w_hi - w_lo) + L - 1) / L;
-        HS_CHECK(hipMemsetAsync(big_count, 0, 4, st));
+        hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, big_count, 1u, 0u);
         hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>());
         hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((chunks + 255) / 256)), blk, 0, st, offsets, sorted,
                            g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+        cx.mark(st, 3);
         hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.bnd.as<hs::xyzz>(), 8u,
                            buckets, big_list, big_count);
         hipLaunchKernelGGL(hs::k_merge_long<C>, dim3(256), dim3(hs::BIG_THREADS), 0, st, offsets, g_lo, L,
                            ws.bnd.as<hs::xyzz>(), big_list, big_count, buckets);
-        cx.mark(st, 3);
+        cx.mark(st, 6);
         const uint32_t nseg = (uint32_t)(w_hi - w_lo) * segs_per_win;
         hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((nseg + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G,
                            (uint32_t)w_lo, nseg, seg);
@@ -511,7 +592,10 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
             hs_gpu_batch_destroy(b);
             return fail(HS_GPU_EINVAL, "batch element " + std::to_string(i) + ": n >= 2^24 (use the plain MSM)");
         }
-        const int c = choose_c_batch(d.n);
+        TuneEntry te;
+        const bool tuned = tune_lookup(1, d.curve, d.n, te);
+        int c = tuned ? te.c : choose_c_batch(d.n);
+        c = c < 4 ? 4 : c > 13 ? 13 : c;
         BatchGroup *g = nullptr;
         for (size_t k = 0; k < keys.size(); k++)
             if (keys[k].first == d.curve && keys[k].second == c)
@@ -524,7 +608,9 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
             g->c = c;
             g->W = (256 + c - 1) / c;
             g->B = 1u << (c - 1);
-            g->G = g->B < 8 ? g->B : 8;
+            const uint32_t G = tuned ? (uint32_t)te.G : 8u;
+            g->G = g->B < G ? g->B : G;
+            g->L = tuned && te.L > 0 ? (uint32_t)te.L : 32u;
             b->groups.push_back(g);
         }
         hs::BatchDesc bd;
@@ -628,7 +714,7 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
     return 0;
 }
 
-const char *const PHASE_NAMES[] = {"ingest", "recode", "sort", "bucket", "reduce", "combine"};
+const char *const PHASE_NAMES[] = {"ingest", "recode", "sort", "bucket", "reduce", "combine", "merge"};
 
 template <class C>
 int map_to_curve_device(Ctx &cx, const uint32_t *d_u, size_t n, uint64_t *d_jac, uint32_t *d_out32, hipStream_t st)
@@ -1030,12 +1116,16 @@ namespace
 
 // Device bytes one request adds to a batch: staged inputs, outputs and its share of every batch
 // buffer (all of which grow linearly with the member MSMs), plus alignment slack.
-size_t request_footprint(size_t n, bool device_inputs)
+size_t request_footprint(int curve, size_t n, bool device_inputs)
 {
     if (n == 0)
         return 1024;
-    const int c = choose_c_batch(n);
-    const size_t W = (256 + c - 1) / c, B = (size_t)1 << (c - 1), G = B < 8 ? B : 8, L = 32;
+    TuneEntry te;
+    const bool tuned = tune_lookup(1, curve, n, te);
+    int c = tuned ? te.c : choose_c_batch(n);
+    c = c < 4 ? 4 : c > 13 ? 13 : c;
+    const size_t W = (256 + c - 1) / c, B = (size_t)1 << (c - 1);
+    const size_t G = std::min(B, (size_t)(tuned ? te.G : 8)), L = tuned && te.L > 0 ? (size_t)te.L : 32;
     const size_t E = n * W, nb = W * B, chunks = (E + L - 1) / L + 1;
     size_t b = sizeof(hs::BatchDesc) + n * 64 + n + E * 2 + E * 4 + 4 + (nb + 1) * 4 + nb * sizeof(hs::xyzz) +
                (nb / G) * sizeof(hs::xyzz) + W * sizeof(hs::xyzz) + sizeof(hs::jac) + chunks * 4 + 4 +
@@ -1183,6 +1273,131 @@ void srv_loop(hs_gpu_server *sp)
     }
 }
 
+
+// ---------------------------------------------------------------- autotuner
+int tune_load_file(const char *path)
+{
+    FILE *f = std::fopen(path, "r");
+    if (!f)
+        return fail(HS_GPU_EINVAL, std::string("cannot open tuning file ") + path);
+    int mode, curve, lg, c, L, G, k = 0;
+    float ms;
+    TuneTable &t = tune();
+    std::lock_guard<std::mutex> lk(t.mu);
+    while (std::fscanf(f, "%d %d %d %d %d %d %f", &mode, &curve, &lg, &c, &L, &G, &ms) == 7)
+    {
+        if (mode < 0 || mode > 1 || curve < 0 || curve > 1 || lg < 0 || lg > 40 || c < 4 || c > 16 || L < 1 || G < 1 ||
+            (G & (G - 1)))
+            continue;
+        TuneEntry e;
+        e.c = c;
+        e.L = L;
+        e.G = G;
+        e.ms = ms;
+        t.e[mode * 1024 + curve * 64 + lg] = e;
+        k++;
+    }
+    std::fclose(f);
+    return k;
+}
+
+// deterministic synthetic inputs on the device: SSWU points of a counter-based byte stream and
+// 255-bit scalars
+struct TuneInputs
+{
+    DevBuf u, pts, sc, out;
+    int make(int curve, size_t n, hipStream_t st)
+    {
+        int r;
+        if ((r = u.reserve(n * 32)) || (r = pts.reserve(n * sizeof(hs_gpu_jacobian))) || (r = sc.reserve(n * 32)) ||
+            (r = out.reserve(4096 * 32)))
+            return r;
+        std::vector<uint32_t> h(n * 8);
+        uint64_t x = 0x9e3779b97f4a7c15ull ^ (uint64_t)n;
+        for (auto &w : h)
+        {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            w = (uint32_t)(x >> 11);
+        }
+        HS_CHECK(hipMemcpy(u.p, h.data(), n * 32, hipMemcpyHostToDevice));
+        for (size_t i = 0; i < n; i++)
+            h[8 * i + 7] &= 0x7fffffffu;
+        HS_CHECK(hipMemcpy(sc.p, h.data(), n * 32, hipMemcpyHostToDevice));
+        Ctx &cx = ctx();
+        if ((r = curve == HS_GPU_HELIOS ? map_to_curve_device<hs::HeliosC>(cx, u.as<uint32_t>(), n, pts.as<uint64_t>(), nullptr, st)
+                                        : map_to_curve_device<hs::SeleneC>(cx, u.as<uint32_t>(), n, pts.as<uint64_t>(), nullptr, st)))
+            return r;
+        HS_CHECK(hipStreamSynchronize(st));
+        return 0;
+    }
+    ~TuneInputs()
+    {
+        for (DevBuf *d : {&u, &pts, &sc, &out})
+            if (d->p)
+                hipFree(d->p);
+    }
+};
+
+// Time (ms, best of `reps`) of one configuration: a plain MSM of n points (mode 0), or a batch of
+// k MSMs of n points each (mode 1).
+int tune_time(int mode, int curve, size_t n, size_t k, size_t ninp, TuneInputs &in, const TuneEntry &cand, int reps,
+              float &best)
+{
+    Ctx &cx = ctx();
+    hipStream_t st = cx.stream;
+    g_force = &cand;
+    int r = 0;
+    hs_gpu_batch *b = nullptr;
+    if (mode == 1)
+    {
+        std::vector<hs_gpu_msm_desc> d(k);
+        for (size_t i = 0; i < k; i++)
+        {
+            d[i].curve = curve;
+            d[i].n = n;
+            const size_t off = (i * n) % (ninp - n + 1); // distinct slices of the inputs
+            d[i].scalars = in.sc.as<unsigned char>() + 32 * off;
+            d[i].points = in.pts.as<hs_gpu_jacobian>() + off;
+            d[i].out32 = in.out.as<unsigned char>() + 32 * (i % 4096);
+            d[i].result = nullptr;
+        }
+        r = batch_create(&b, d.data(), k, 0);
+    }
+    g_force = nullptr;
+    if (r)
+        return r;
+    hipEvent_t e0, e1;
+    HS_CHECK(hipEventCreate(&e0));
+    HS_CHECK(hipEventCreate(&e1));
+    best = 1e30f;
+    for (int i = 0; i <= reps && !r; i++)
+    {
+        HS_CHECK(hipEventRecord(e0, st));
+        if (mode == 0)
+        {
+            g_force = &cand;
+            std::lock_guard<std::mutex> lk(cx.mu);
+            r = msm_device_any(curve, cx, in.sc.p, in.pts.p, n, 0, in.out.p, nullptr, st);
+            g_force = nullptr;
+        }
+        else
+            r = batch_enqueue(*b, st);
+        HS_CHECK(hipEventRecord(e1, st));
+        HS_CHECK(hipEventSynchronize(e1));
+        float ms;
+        HS_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (i > 0 && ms < best) // the first run warms the workspace
+            best = ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (b)
+        hs_gpu_batch_destroy(b);
+    return r;
+}
+
 } // namespace
 
 extern "C"
@@ -1243,7 +1458,7 @@ int hs_gpu_server_submit(hs_gpu_server *s, const hs_gpu_msm_desc *req, int devic
     auto *q = new SrvReq();
     q->d = *req;
     q->device_inputs = device_inputs != 0;
-    q->foot = request_footprint(req->n, q->device_inputs);
+    q->foot = request_footprint(req->curve, req->n, q->device_inputs);
     q->t_submit = std::chrono::steady_clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -1797,11 +2012,7 @@ int hs_gpu_msm_bytes(int curve, unsigned char out32[32], const unsigned char *sc
 
 const char *hs_gpu_version(void) { return "hs_gpu 0.1 gfx950"; }
 
-int hs_gpu_window_bits(int curve, size_t n)
-{
-    (void)curve;
-    return choose_c(n);
-}
+int hs_gpu_window_bits(int curve, size_t n) { return make_plan(curve, n).c; }
 
 int hs_gpu_reference_window_bits(size_t n) { return ref_window(n); }
 
@@ -1959,6 +2170,130 @@ void hs_gpu_selene_pedersen_commit(hs_gpu_jacobian *result, const unsigned char 
                                    const unsigned char *values, const hs_gpu_jacobian *generators, size_t n)
 {
     slot_pedersen(HS_GPU_SELENE, result, blinding, H, values, generators, n);
+}
+
+
+int hs_gpu_autotune(int curve, int mode, int lg_lo, int lg_hi, const char *path)
+{
+    if ((curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE) || mode < 0 || mode > 1 || lg_lo < 6 || lg_hi > 22 ||
+        lg_lo > lg_hi)
+        return fail(HS_GPU_EINVAL, "autotune: bad curve, mode or size range (2^6 .. 2^22)");
+    Ctx &cx = ctx();
+    int r;
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if ((r = cx.init()))
+            return r;
+    }
+    const int cmax = mode == 0 ? 16 : 13;
+    TuneInputs in;
+    const size_t nmax = (size_t)1 << lg_hi;
+    // a batch of MSMs of 2^lg points reads the same 2^18-point inputs (or all of them, if larger)
+    const size_t ninp = mode == 0 ? nmax : std::max(nmax, (size_t)1 << 18);
+    if ((r = in.make(curve, ninp, cx.stream)))
+        return r;
+    for (int lg = lg_lo; lg <= lg_hi; lg++)
+    {
+        const size_t n = (size_t)1 << lg;
+        const size_t k = mode == 0 ? 1 : std::max<size_t>(4, std::min<size_t>(256, ((size_t)1 << 19) >> lg));
+        TuneEntry best;
+        best.c = mode == 0 ? choose_c(n) : choose_c_batch(n);
+        best.L = 32;
+        best.G = 8;
+        if ((r = tune_time(mode, curve, n, k, ninp, in, best, 2, best.ms)))
+            return r;
+        // coordinate descent: window bits, then run length, then segment size
+        auto try_cand = [&](TuneEntry cand) -> int {
+            float ms;
+            int rr = tune_time(mode, curve, n, k, ninp, in, cand, 2, ms);
+            if (!rr && ms < best.ms)
+            {
+                best = cand;
+                best.ms = ms;
+            }
+            return rr;
+        };
+        const int c0 = best.c;
+        for (int dc : {-2, -1, 1})
+        {
+            TuneEntry cand = best;
+            cand.c = c0 + dc;
+            if (cand.c >= 4 && cand.c <= cmax && (r = try_cand(cand)))
+                return r;
+        }
+        const int L0 = best.L;
+        for (int L : {16, 64})
+        {
+            TuneEntry cand = best;
+            cand.L = L;
+            if (L != L0 && (r = try_cand(cand)))
+                return r;
+        }
+        for (int G : {4, 16})
+        {
+            TuneEntry cand = best;
+            cand.G = G;
+            if ((r = try_cand(cand)))
+                return r;
+        }
+        {
+            TuneTable &t = tune();
+            std::lock_guard<std::mutex> lk(t.mu);
+            t.loaded_env = true;
+            t.e[mode * 1024 + curve * 64 + lg] = best;
+        }
+        if (path)
+        {
+            FILE *f = std::fopen(path, "a");
+            if (!f)
+                return fail(HS_GPU_EINVAL, std::string("cannot write tuning file ") + path);
+            std::fprintf(f, "%d %d %d %d %d %d %.4f\n", mode, curve, lg, best.c, best.L, best.G, best.ms);
+            std::fclose(f);
+        }
+    }
+    return 0;
+}
+
+int hs_gpu_tune_load(const char *path)
+{
+    if (!path)
+        return fail(HS_GPU_EINVAL, "null path");
+    const int k = tune_load_file(path);
+    {
+        TuneTable &t = tune();
+        std::lock_guard<std::mutex> lk(t.mu);
+        t.loaded_env = true;
+    }
+    return k < 0 ? k : 0;
+}
+
+int hs_gpu_tune_get(int curve, int mode, size_t n, int *c, int *L, int *G)
+{
+    if (!c || !L || !G)
+        return fail(HS_GPU_EINVAL, "null argument");
+    TuneEntry te;
+    if (mode == 0)
+    {
+        const Plan p = make_plan(curve, n);
+        *c = p.c;
+        *L = (int)p.L;
+        *G = (int)p.G;
+        return 0;
+    }
+    const bool tuned = tune_lookup(1, curve, n, te);
+    *c = tuned ? te.c : choose_c_batch(n);
+    *L = tuned ? te.L : 32;
+    *G = tuned ? te.G : 8;
+    return 0;
+}
+
+int hs_gpu_tune_clear(void)
+{
+    TuneTable &t = tune();
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.e.clear();
+    t.loaded_env = true;
+    return 0;
 }
 
 } // extern "C"

@@ -211,6 +211,24 @@ int hs_gpu_server_get_stats(hs_gpu_server *server, hs_gpu_server_stats *out);
 int hs_gpu_server_reset_stats(hs_gpu_server *server);
 void hs_gpu_server_destroy(hs_gpu_server *server); /* drains queued and running requests first */
 
+/* ---------------------------------------------------------------- autotuner
+ * No reference counterpart (the reference's autotune picks CPU backends, src/dispatch.cpp:200-360).
+ * hs_gpu_autotune measures, on this device, candidate plan parameters for MSMs of 2^lg points for
+ * each lg in [lg_lo, lg_hi] (6 <= lg_lo <= lg_hi <= 22): the window width c, the run length L of
+ * the flat bucket kernel (sorted entries per thread) and the reduction segment G (buckets per
+ * running-sum segment).  mode 0 tunes the plain MSM (latency of one MSM); mode 1 tunes a group of
+ * a batch (time of many MSMs of that size in one batch).  It searches coordinate-wise (c, then L,
+ * then G) from the built-in defaults, installs the fastest into the library's table (used by all
+ * later plans, batches and server admissions of that size class +-1) and appends lines
+ * "mode curve lg c L G ms" to `path` when it is not NULL.  hs_gpu_tune_load reads such a file (the
+ * environment variable HS_GPU_TUNE_FILE names one to load at first use); hs_gpu_tune_get reports
+ * the parameters a plan of n points would use; hs_gpu_tune_clear restores the defaults.
+ * Tuning does not change results, only speed. */
+int hs_gpu_autotune(int curve, int mode, int lg_lo, int lg_hi, const char *path);
+int hs_gpu_tune_load(const char *path);
+int hs_gpu_tune_get(int curve, int mode, size_t n, int *c, int *L, int *G);
+int hs_gpu_tune_clear(void);
+
 /* ---------------------------------------------------------------- hash-to-curve
  * Simplified SWU map of n 32-byte inputs (RFC 9380 6.6.2; reference helios_map_to_curve /
  * selene_map_to_curve, helios/src/x64/helios_map_to_curve.cpp:184-189).  Outputs the reference
