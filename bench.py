@@ -52,6 +52,20 @@ def msm_int32_ops(curve, n):
     return 2 * (best + 256 * (3 * m + 5 * s))
 
 
+def measured_traffic(n):
+    """HBM bytes per k_bucket_flat launch from the committed counter summary (two rocprofv3 --pmc
+    runs, FETCH_SIZE and WRITE_SIZE, scaled by a calibration kernel of known size), or None."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "plain_msm_n2e20_kernel_trace.md")
+    if n != 1 << 20 or not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch: helios=(\d+) selene=(\d+)", open(path).read())
+    if not m:
+        return None
+    return {"helios": int(m.group(1)), "selene": int(m.group(2)), "source": os.path.relpath(path, ROOT)}
+
+
 def make_inputs(hs, torch, curve, n, seed, dev):
     import numpy as np
 
@@ -364,6 +378,7 @@ def main():
     ap.add_argument("--serve-bytes", type=int, default=16, help="server device budget, GiB")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in plan parameters)")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams")
     args = ap.parse_args()
@@ -381,6 +396,21 @@ def main():
     torch.cuda.init()
     streams = [torch.cuda.Stream() for _ in range(args.streams)]
     n = args.n
+    tuning = None
+    if not args.no_tune:
+        # the library's autotuner picks (window bits, run length, segment size) per size class on
+        # this device: plain MSMs at the headline size, batch groups over the batched / served /
+        # Pedersen size range
+        t_tune = time.perf_counter()
+        lg = max(6, min(22, n.bit_length() - 1))
+        for c in (0, 1):
+            hs.autotune(c, 0, lg, lg)
+            hs.autotune(c, 1, 6, 18)
+        tuning = {"seconds": time.perf_counter() - t_tune,
+                  "plain": {cv: hs.tune_get(c, 0, n) for c, cv in enumerate(("helios", "selene"))},
+                  "batch": {cv: {f"2^{k}": hs.tune_get(c, 1, 1 << k) for k in range(6, 19)}
+                            for c, cv in enumerate(("helios", "selene"))},
+                  "fields": "(window bits c, entries per thread L, buckets per segment G)"}
     with torch.cuda.stream(streams[0]):
         inputs = [make_inputs(hs, torch, c, n, args.seed + 17 * c, dev) for c in (0, 1)]
         out = torch.zeros((2, 8), dtype=torch.int32, device=dev)
@@ -400,11 +430,11 @@ def main():
             step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        # HBM counter calibration: one plain 256 MiB device copy (read 256 MiB, write 256 MiB), so a
-        # FETCH_SIZE / WRITE_SIZE counter run of this command can scale the MSM kernels' counts
-        cal = torch.empty(1 << 26, dtype=torch.int32, device=dev)
+        # HBM counter calibration: one elementwise kernel that reads 256 MiB and writes 256 MiB, so
+        # a FETCH_SIZE / WRITE_SIZE counter run of this command can scale the MSM kernels' counts
+        cal = torch.zeros(1 << 26, dtype=torch.int32, device=dev)
         cal2 = torch.empty_like(cal)
-        cal2.copy_(cal)
+        torch.add(cal, 1, out=cal2)
         torch.cuda.synchronize()
         del cal, cal2
         # roofline pass: the same MSMs one at a time on one stream, HIP events between phases
@@ -428,6 +458,7 @@ def main():
                     for c in (0, 1)}
     msm_ms = [sum(per_curve_ms[cv].values()) for cv in ("helios", "selene")]
     msm_ops = [msm_int32_ops(c, n) for c in (0, 1)]
+    traffic = measured_traffic(n)
 
     rec = {
         "metric": METRIC,
@@ -458,7 +489,9 @@ def main():
             "peak": peak / 1e12,
             "unit": "TOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": (None if traffic is None else (traffic["helios"] + traffic["selene"]) / 2),
+            "traffic_per_launch": traffic,
+            "algorithmic_bytes_per_launch": -(-256 // cstar[0]) * n * (4 + 64),
             "algorithmic_ops_per_launch": {"helios": ops[0], "selene": ops[1], "c_star": list(cstar)},
             "kernel_ms": {"helios": bucket_ms[0], "selene": bucket_ms[1]},
             "kernel_ms_source": "HIP events around k_chunk_start + k_bucket_flat on the MSM's stream, "
@@ -468,6 +501,7 @@ def main():
                                "frac": msm_ops[c] / (msm_ms[c] * 1e-3) / peak}
                           for c, cv in enumerate(("helios", "selene"))},
         },
+        "autotune": tuning,
         "phase_ms_isolated": per_curve_ms,
         "per_curve_points_per_s_isolated": {cv: n / (msm_ms[c] * 1e-3) for c, cv in enumerate(("helios", "selene"))},
     }

@@ -39,6 +39,7 @@ struct Fp
     static constexpr int NR = 1; // limbs of R = 2^261 mod p
     static HS_HD uint32_t mod(int i) { return i == 0 ? 0x1fffffedu : i == 8 ? 0x7fffffu : 0x1fffffffu; }
     static HS_HD uint32_t r(int i) { return i == 0 ? 1216u : 0u; }
+    static HS_HD uint32_t g(int i) { return i == 0 ? 19u : 0u; }
     // 129 * p with every limb in [2^30, 2^31): subtraction bias
     static HS_HD uint32_t bias(int i)
     {
@@ -76,6 +77,19 @@ struct Fq
         case 2: return 0x094a7914u;
         case 3: return 0x03e9a44du;
         case 4: return 0x00010202u;
+        default: return 0u;
+        }
+    }
+    // gamma = 2^255 mod q in 29-bit limbs (127 bits)
+    static HS_HD uint32_t g(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x06d83861u;
+        case 1: return 0x0a496c6cu;
+        case 2: return 0x06a529e4u;
+        case 3: return 0x010fa691u;
+        case 4: return 0x00000408u;
         default: return 0u;
         }
     }
@@ -183,7 +197,30 @@ template <class F> HS_HD void fe_norm(fe &r)
         r.v[i] = t & M29;
         c = t >> 29;
     }
-    fold_top<F>(r, c);
+    if (F::NR == 1)
+        fold_top<F>(r, c);
+    else
+    {
+        // fold everything from bit 255 up: value = lo + 2^255 top = lo + gamma top, top < 2^9,
+        // so the sum stays below 2^255 + 2^136 and one carry pass leaves every limb < 2^29
+        const uint32_t top = (r.v[NL - 1] >> 23) | (c << 6);
+        r.v[NL - 1] &= (1u << 23) - 1;
+        uint64_t u = 0;
+#pragma unroll
+        for (int i = 0; i < 5; i++)
+        {
+            u = mad64(top, F::g(i), (uint64_t)r.v[i] + (u >> 29));
+            r.v[i] = (uint32_t)u & M29;
+        }
+        uint32_t cc = (uint32_t)(u >> 29);
+#pragma unroll
+        for (int i = 5; i < NL; i++)
+        {
+            const uint32_t t = r.v[i] + cc;
+            r.v[i] = t & M29;
+            cc = t >> 29;
+        }
+    }
 }
 
 template <class F> HS_HD void fe_add(fe &r, const fe &a, const fe &b)
