@@ -379,8 +379,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in plan parameters)")
-    ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
-                    help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams")
+    ap.add_argument("--streams", type=int, default=4, choices=(1, 2, 4),
+                    help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams; "
+                         "4: consecutive steps alternate between two such stream pairs (pipelined steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (each MSM fits one "
@@ -413,21 +414,24 @@ def main():
                   "fields": "(window bits c, entries per thread L, buckets per segment G)"}
     with torch.cuda.stream(streams[0]):
         inputs = [make_inputs(hs, torch, c, n, args.seed + 17 * c, dev) for c in (0, 1)]
-        out = torch.zeros((2, 8), dtype=torch.int32, device=dev)
+        out = torch.zeros((4, 8), dtype=torch.int32, device=dev)
         torch.cuda.synchronize()
-        sts = [streams[c % args.streams].cuda_stream for c in (0, 1)]
+        pairs = [[streams[min(c, args.streams - 1)].cuda_stream for c in (0, 1)]] if args.streams < 4 else \
+            [[streams[0].cuda_stream, streams[1].cuda_stream], [streams[2].cuda_stream, streams[3].cuda_stream]]
+        sts = pairs[0]
 
-        def step():
+        def step(i):
             for c in (0, 1):
                 sc, pts = inputs[c]
-                hs.msm_device(c, sc.data_ptr(), pts.data_ptr(), n, out[c].data_ptr(), None, 0, sts[c])
+                k = i % len(pairs)
+                hs.msm_device(c, sc.data_ptr(), pts.data_ptr(), n, out[2 * k + c].data_ptr(), None, 0, pairs[k][c])
 
-        for _ in range(args.warmup):
-            step()
+        for i in range(args.warmup):
+            step(i)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
+        for i in range(args.steps):
+            step(i)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         # HBM counter calibration: one elementwise kernel that reads 256 MiB and writes 256 MiB, so
