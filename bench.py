@@ -1,17 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark of the plain variable-time Pippenger MSM on one MI355X.
+"""Benchmark of the variable-time Pippenger MSM on one MI355X (one JSON line).
 
-One step = one Helios MSM and one Selene MSM of n = 2^20 points each (the single-MSM GPU
-workload of the Helios/Selene MSM path), on inputs already resident in HBM:
+Headline (`value`): plain single MSMs.  One step = one Helios MSM and one Selene MSM of n = 2^20
+points each, on inputs already resident in HBM:
   * bases: SSWU hash-to-curve images of seeded uniform 32-byte strings (computed on the GPU by
     the library's own map), with 0.5% of them duplicates of another base and 0.1% the identity;
   * scalars: uniform modulo each curve's group order, 1% forced to edge values
     (0, 1, order - 1, a small 8-bit value).
-Prints one JSON line.  `value` is points per second over the timed steps (2 * n points per
-step).  `roofline` reports the bucket-accumulation kernel (the dominant one) against the
-device's 32x32->64 multiply throughput, measured in the same run by the library's probe;
-`cpu_baseline` times the reference's own CPU MSM (compiled from its sources into
-oracle/_ref/libhsref.so) on a bounded sample of the same inputs.
+Helios and Selene run on their own streams and consecutive steps alternate between two stream
+pairs, so `value` (points per second over the timed steps, 2 * n points per step) is the
+throughput of a stream of plain MSMs.  Before measuring, the library's autotuner picks plan
+parameters per size class on this device (--no-tune: built-in defaults).
+
+The same line carries:
+  * `roofline`: k_bucket_flat (the dominant kernel) against the device's measured 32x32->64
+    multiply throughput, from HIP events in an isolated pass (one MSM at a time), plus the whole
+    MSM's fraction and the kernel's HBM traffic from the committed counter summary in profiles/;
+  * `batched_mix`: 256 Helios + 256 Selene MSMs of log-uniform sizes in [64, 2^14], plain
+    per-MSM launches against one batch plan replayed from a hipGraph;
+  * `pedersen_batch`: 1024 commitments per curve over H + 2^12 resident generators, both
+    fixed-base methods;
+  * `serving`: Poisson request stream through the native server against plain per-request
+    launches, p50/p99 latency and saturated throughput;
+  * `host_slot_points_per_s`: the drop-in slot with host buffers (PCIe upload included);
+  * `cpu_baseline`: the reference's own CPU MSM (compiled from its sources into
+    oracle/_ref/libhsref.so) on a bounded sample of the same inputs, one thread.
 """
 import argparse
 import ctypes

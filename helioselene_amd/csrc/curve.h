@@ -135,10 +135,8 @@ template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
     fe_mul<F>(Q, acc.X, PP);
     fe X3, Y3;
     fe_sq<F>(X3, R);
-    fe_sub<F>(X3, X3, PPP);
-    fe_sub<F>(X3, X3, Q);
-    fe_sub<F>(X3, X3, Q);
-    fe_sub<F>(t, Q, X3);
+    fe_sub3<F>(X3, X3, PPP, Q, Q); // R^2 - PPP - 2Q
+    fe_sub_lazy<F>(t, Q, X3);      // only multiplied by the normalised R
     fe_mul<F>(Y3, R, t);
     fe_mul<F>(t, acc.Y, PPP);
     fe_sub<F>(Y3, Y3, t);
@@ -182,10 +180,8 @@ template <class F> HS_HD void xyzz_add(xyzz &r, const xyzz &p, const xyzz &q)
     fe_mul<F>(Q, U1, PP);
     fe X3, Y3;
     fe_sq<F>(X3, R);
-    fe_sub<F>(X3, X3, PPP);
-    fe_sub<F>(X3, X3, Q);
-    fe_sub<F>(X3, X3, Q);
-    fe_sub<F>(t, Q, X3);
+    fe_sub3<F>(X3, X3, PPP, Q, Q); // R^2 - PPP - 2Q
+    fe_sub_lazy<F>(t, Q, X3);      // only multiplied by the normalised R
     fe_mul<F>(Y3, R, t);
     fe_mul<F>(t, S1, PPP);
     fe_sub<F>(Y3, Y3, t);

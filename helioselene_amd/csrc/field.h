@@ -45,6 +45,8 @@ struct Fp
     {
         return i == 0 ? 0x5ffff66du : i == 8 ? 0x407ffffdu : 0x5ffffffdu;
     }
+    // 257 p with every limb in [2^31, 2^31 + 2^29): bias of a subtraction of three normalised values
+    static HS_HD uint32_t bias3(int i) { return i == 0 ? 0x9fffecedu : i == 8 ? 0x807ffffbu : 0x9ffffffbu; }
     // modulus and 2^255 mod m as 32-bit words (canonicalisation)
     static HS_HD uint32_t mw(int i) { return i == 0 ? 0xffffffedu : i == 7 ? 0x7fffffffu : 0xffffffffu; }
     static HS_HD uint32_t cw(int i) { return i == 0 ? 19u : 0u; }
@@ -104,6 +106,19 @@ struct Fq
         case 4: return 0x5ffdf7f1u;
         case 8: return 0x407ffffdu;
         default: return 0x5ffffffdu;
+        }
+    }
+    static HS_HD uint32_t bias3(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x80ef669fu;
+        case 1: return 0x8c4a2759u;
+        case 2: return 0x9430f1c5u;
+        case 3: return 0x8f49c835u;
+        case 4: return 0x9ffbf3ebu;
+        case 8: return 0x807ffffbu;
+        default: return 0x9ffffffbu;
         }
     }
     // q = 2^255 - gamma, gamma = 0x408087d3489a94a791492d8d86d83861 (reference fq51.h:39-62)
@@ -239,6 +254,25 @@ template <class F> HS_HD void fe_sub(fe &r, const fe &a, const fe &b)
     fe_norm<F>(r);
 }
 
+// r = a - b - c - d for normalised inputs, one normalisation (bias 257 m, limbs < 2^31.6)
+template <class F> HS_HD void fe_sub3(fe &r, const fe &a, const fe &b, const fe &c, const fe &d)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = a.v[i] + F::bias3(i) - b.v[i] - c.v[i] - d.v[i];
+    fe_norm<F>(r);
+}
+
+// r = a - b without normalisation: limbs < 2^29 + 2^31, value < 2^262.3.  Only valid as one
+// operand of fe_mul whose other operand is normalised (column sums stay below 2^63.6, inside the
+// bounds reduce_cols is written for); never as an input of fe_sq, fe_add or fe_sub.
+template <class F> HS_HD void fe_sub_lazy(fe &r, const fe &a, const fe &b)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = a.v[i] + F::bias(i) - b.v[i];
+}
+
 template <class F> HS_HD void fe_neg(fe &r, const fe &a)
 {
     fe z;
@@ -248,7 +282,9 @@ template <class F> HS_HD void fe_neg(fe &r, const fe &a)
 
 template <class F> HS_HD void fe_dbl(fe &r, const fe &a) { fe_add<F>(r, a, a); }
 
-// 17 product columns (each < 9 * 2^58 < 2^61.2) -> normalised residue
+// 17 product columns -> normalised residue.  Columns are < 9 * 2^58 < 2^61.2 for normalised
+// operands and < 9 * 2^29 * (2^29 + 2^31) < 2^63.6 when one operand comes from fe_sub_lazy; every
+// intermediate below is sized for the latter.
 template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
 {
     if (F::NR == 1)

@@ -192,6 +192,45 @@ template <class F> static void pop(int o, uint8_t *out, const uint8_t *pa, const
     out[64] = ok;
 }
 
+// sum of n affine points (x || y, 64 bytes each; all-zero = identity, skipped) accumulated with
+// xyzz_madd in order, as the bucket kernels do; neg[i] != 0 adds -P_i
+template <class F> static void madd_chain(uint8_t *out, const uint8_t *pts, const uint8_t *neg, size_t n)
+{
+    xyzz acc;
+    bool aid = true;
+    xyzz_identity<F>(acc);
+    for (size_t i = 0; i < n; i++)
+    {
+        int nz = 0;
+        for (int j = 0; j < 64; j++)
+            nz |= pts[64 * i + j];
+        if (!nz)
+            continue;
+        aff q;
+        ld(q.x, pts + 64 * i);
+        ld(q.y, pts + 64 * i + 32);
+        if (neg[i])
+            fe_neg<F>(q.y, q.y);
+        xyzz_madd<F>(acc, aid, q);
+    }
+    if (aid)
+        xyzz_identity<F>(acc);
+    aff r;
+    bool ok = xyzz_to_aff<F>(r, acc);
+    st(out, r.x);
+    st(out + 32, r.y);
+    out[64] = ok;
+}
+
+extern "C" void hst_madd_chain(int field, uint8_t *out, const uint8_t *pts, const uint8_t *neg, size_t n)
+{
+    g_field = field;
+    if (field == 0)
+        madd_chain<Fp>(out, pts, neg, n);
+    else
+        madd_chain<Fq>(out, pts, neg, n);
+}
+
 extern "C" void hst_pt_op(int field, int o, uint8_t *out, const uint8_t *a, const uint8_t *b, uint32_t k)
 {
     g_field = field;

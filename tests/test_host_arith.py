@@ -101,6 +101,40 @@ def test_point_ops_vs_oracle(L, c):
 
 
 @pytest.mark.parametrize("c", [0, 1])
+def test_madd_chain_vs_oracle(L, c):
+    """Long xyzz_madd chains (the bucket accumulation's inner loop) with repeated points, negated
+    points, identities and cancellations, against the oracle's sum."""
+    rnd = random.Random(40 + c)
+    g = O.generator(c)
+    base = [O.scalar_mul(c, rnd.getrandbits(250).to_bytes(32, "little"), g) for _ in range(40)]
+    for trial in range(6):
+        n = 300
+        idx = [rnd.randrange(len(base)) for _ in range(n)]
+        neg = [rnd.random() < 0.5 for _ in range(n)]
+        if trial % 2:
+            # force doublings (acc == P) and cancellations (acc == -P) mid-chain
+            idx[10:14] = [idx[9]] * 4
+            neg[10:14] = [neg[9]] * 4
+            idx[50], neg[50] = idx[49], not neg[49]
+        pts, want_sc = bytearray(), bytearray()
+        for i in range(n):
+            P = base[idx[i]]
+            x, y = GH.decompress(c, P)
+            pts += x.to_bytes(32, "little") + y.to_bytes(32, "little")
+        pts[64 * 7:64 * 8] = bytes(64)  # one identity entry
+        sc = [0] * len(base)
+        for i in range(n):
+            if i != 7:
+                sc[idx[i]] += -1 if neg[i] else 1
+        order = O.ORDER[c]
+        want = O.msm_naive(c, b"".join((v % order).to_bytes(32, "little") for v in sc), b"".join(base), len(base))
+        out = ctypes.create_string_buffer(65)
+        L.hst_madd_chain.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.hst_madd_chain(c, out, bytes(pts), bytes(int(v) for v in neg), n)
+        assert _enc(out.raw) == want, trial
+
+
+@pytest.mark.parametrize("c", [0, 1])
 def test_decompress_and_sswu_vs_oracle(L, c, golden):
     """Device decode / hash-to-curve code on the host vs the oracle and the reference vectors."""
     v = golden[0][O.CURVES[c] + "_point"]
@@ -191,6 +225,18 @@ def test_field_extreme_limbs(L, k):
             assert out.raw[39] == 0, ("limb bound", o, la, lb)
             v = int.from_bytes(out.raw[:39], "little")
             assert v < (1 << 262) and v % m == want % m, (o, la, lb)
+    # products with one lazily subtracted operand (limbs up to 2^29 + 2^31) and one normalised
+    lazy_top = top + (1 << 31) - 1
+    lazy = [[lazy_top] * 9, [lazy_top, top] * 4 + [lazy_top]] + \
+        [[rnd.randrange(0, lazy_top + 1) for _ in range(9)] for _ in range(200)]
+    for i, la in enumerate(lazy):
+        lb = pats[i % len(pats)]
+        a, b = _limbs_value(la), _limbs_value(lb)
+        for x, y in ((la, lb), (lb, la)):
+            L.hst_fe_op_limbs(k, 0, out, (ctypes.c_uint32 * 9)(*x), (ctypes.c_uint32 * 9)(*y))
+            assert out.raw[39] == 0, ("limb bound", la, lb)
+            v = int.from_bytes(out.raw[:39], "little")
+            assert v < (1 << 262) and v % m == (a * b) % m, (la, lb)
     # zero test: j*m for j = 0..64 in strict digits, and with one digit borrowed into the next
     for j in list(range(65)):
         v = j * m
