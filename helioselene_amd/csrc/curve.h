@@ -53,10 +53,9 @@ template <class F> HS_HD void tangent_m(fe &m, const fe &x, const fe &z)
 {
     fe t0, t1;
     fe_sub<F>(t0, x, z);
-    fe_add<F>(t1, x, z);
-    fe_mul<F>(m, t0, t1);
-    fe_add<F>(t0, m, m);
-    fe_add<F>(m, t0, m);
+    fe_add_lazy<F>(t1, x, z); // multiplied by the normalised t0 only
+    fe_mul<F>(t0, t0, t1);
+    fe_mul_small<F>(m, t0, 3);
 }
 
 // dbl-2008-s-1 with a = -3 (M = 3X^2 - 3ZZ^2 = 3(X-ZZ)(X+ZZ)); 7M + 2S
@@ -70,9 +69,8 @@ template <class F> HS_HD void xyzz_dbl(xyzz &r, const xyzz &p)
     tangent_m<F>(M, p.X, p.ZZ);
     fe X3, Y3;
     fe_sq<F>(X3, M);
-    fe_sub<F>(X3, X3, S);
-    fe_sub<F>(X3, X3, S);
-    fe_sub<F>(t, S, X3);
+    fe_sub2<F>(X3, X3, S, S); // M^2 - 2S
+    fe_sub_lazy<F>(t, S, X3); // only multiplied by the normalised M
     fe_mul<F>(Y3, M, t);
     fe_mul<F>(t, W, p.Y);
     fe_sub<F>(Y3, Y3, t);
@@ -94,9 +92,8 @@ template <class F> HS_HD void xyzz_mdbl(xyzz &r, const aff &p)
     tangent_m<F>(M, p.x, one);
     fe X3, Y3;
     fe_sq<F>(X3, M);
-    fe_sub<F>(X3, X3, S);
-    fe_sub<F>(X3, X3, S);
-    fe_sub<F>(t, S, X3);
+    fe_sub2<F>(X3, X3, S, S); // M^2 - 2S
+    fe_sub_lazy<F>(t, S, X3); // only multiplied by the normalised M
     fe_mul<F>(Y3, M, t);
     fe_mul<F>(t, W, p.y);
     fe_sub<F>(Y3, Y3, t);

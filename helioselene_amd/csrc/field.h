@@ -181,44 +181,10 @@ template <class F> HS_HD void fold_top(fe &r, uint32_t c)
     }
     else
     {
-        // two passes: the second carry (0 or 1) cannot ripple past limb 5
-        uint64_t t = 0;
-#pragma unroll
-        for (int i = 0; i < NL; i++)
-        {
-            t = (i < F::NR ? mad64(c, F::r(i), r.v[i]) : (uint64_t)r.v[i]) + (t >> 29);
-            r.v[i] = (uint32_t)t & M29;
-        }
-        uint32_t c2 = (uint32_t)(t >> 29);
-        uint32_t u = 0;
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-        {
-            u = (i < F::NR ? c2 * F::r(i) + r.v[i] : r.v[i]) + (u >> 29);
-            r.v[i] = u & M29;
-        }
-        r.v[6] += u >> 29;
-    }
-}
-
-// limbs < 2^31.7 -> normalised
-template <class F> HS_HD void fe_norm(fe &r)
-{
-    uint32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-    {
-        uint32_t t = r.v[i] + c;
-        r.v[i] = t & M29;
-        c = t >> 29;
-    }
-    if (F::NR == 1)
-        fold_top<F>(r, c);
-    else
-    {
-        // fold everything from bit 255 up: value = lo + 2^255 top = lo + gamma top, top < 2^9,
-        // so the sum stays below 2^255 + 2^136 and one carry pass leaves every limb < 2^29
-        const uint32_t top = (r.v[NL - 1] >> 23) | (c << 6);
+        // fold everything from bit 255 up: value = lo + 2^255 top = lo + gamma top with
+        // top = bits 255.. of limb 8 plus c * 2^6 (< 2^20), so the sum stays below
+        // 2^255 + 2^147 and one carry pass leaves every limb < 2^29
+        const uint32_t top = (r.v[NL - 1] >> 23) + (c << 6);
         r.v[NL - 1] &= (1u << 23) - 1;
         uint64_t u = 0;
 #pragma unroll
@@ -236,6 +202,20 @@ template <class F> HS_HD void fe_norm(fe &r)
             cc = t >> 29;
         }
     }
+}
+
+// limbs < 2^31.7 -> normalised
+template <class F> HS_HD void fe_norm(fe &r)
+{
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        uint32_t t = r.v[i] + c;
+        r.v[i] = t & M29;
+        c = t >> 29;
+    }
+    fold_top<F>(r, c);
 }
 
 template <class F> HS_HD void fe_add(fe &r, const fe &a, const fe &b)
@@ -263,6 +243,15 @@ template <class F> HS_HD void fe_sub3(fe &r, const fe &a, const fe &b, const fe 
     fe_norm<F>(r);
 }
 
+// r = a - b - c for normalised inputs, one normalisation
+template <class F> HS_HD void fe_sub2(fe &r, const fe &a, const fe &b, const fe &c)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = a.v[i] + F::bias3(i) - b.v[i] - c.v[i];
+    fe_norm<F>(r);
+}
+
 // r = a - b without normalisation: limbs < 2^29 + 2^31, value < 2^262.3.  Only valid as one
 // operand of fe_mul whose other operand is normalised (column sums stay below 2^63.6, inside the
 // bounds reduce_cols is written for); never as an input of fe_sq, fe_add or fe_sub.
@@ -271,6 +260,14 @@ template <class F> HS_HD void fe_sub_lazy(fe &r, const fe &a, const fe &b)
 #pragma unroll
     for (int i = 0; i < NL; i++)
         r.v[i] = a.v[i] + F::bias(i) - b.v[i];
+}
+
+// r = a + b without normalisation (limbs <= 2^30): as one operand of fe_mul only, like fe_sub_lazy
+template <class F> HS_HD void fe_add_lazy(fe &r, const fe &a, const fe &b)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = a.v[i] + b.v[i];
 }
 
 template <class F> HS_HD void fe_neg(fe &r, const fe &a)
