@@ -34,6 +34,20 @@ def test_batch_mixed_sizes_and_curves():
         assert GH.jac_to_bytes(items[k][0], jac) == w
 
 
+def test_batch_random_z_points():
+    """Batch ingest with random Z (lazy limbs) mixed with Z = 1 and identities across MSMs."""
+    rnd = random.Random(3)
+    items, want = [], []
+    for k, n in enumerate((70, 3000, 5000, 1)):
+        c = k % 2
+        sc, pb, _ = _case(c, n, 400 + k)
+        pts = GH.to_jac_array(c, pb, n, rnd=rnd, lazy=True) if k % 3 != 2 else GH.to_jac_array(c, pb, n)
+        items.append((c, GH.scalars_np(sc, n), pts))
+        want.append(O.msm(c, sc, pb, n))
+    got = hs.msm_batch(items)
+    assert [g[0] for g in got] == want
+
+
 def test_batch_exact_mode_window8():
     """Exact mode in a batch: elements in the reference window-8 band equal the naive sum."""
     items, want = [], []

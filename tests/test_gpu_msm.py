@@ -81,6 +81,25 @@ def test_reference_run_4096_kG(golden):
 
 
 @pytest.mark.parametrize("c", [0, 1])
+def test_mixed_z_ingest(c):
+    """Several ingest blocks with every kind of Z mixed within one thread's points (random Z with
+    lazy limbs, Z = 1, identities): the shared batched inversion must give each point its own
+    affine form."""
+    n = 6000
+    rnd = random.Random(77 + c)
+    sc = O.gen_scalars(c, 17, n, edge_frac=0.01)
+    pb = O.gen_sswu_bases(c, 19, n, dup_frac=0.01, id_frac=0.02)
+    rz = GH.to_jac_array(c, pb, n, rnd=rnd, lazy=True)
+    z1 = GH.to_jac_array(c, pb, n)
+    pick = np.array([rnd.random() < 0.4 for _ in range(n)])
+    pts = np.where(pick[:, None], z1, rz)
+    out, _ = run(c, sc, pts)
+    assert out == O.msm(c, sc, pb, n)
+    allz1, _ = run(c, sc, z1)
+    assert allz1 == out
+
+
+@pytest.mark.parametrize("c", [0, 1])
 @pytest.mark.parametrize("n", [1, 2, 3, 16, 17, 31, 100, 700])
 def test_random_vs_oracle(c, n):
     """Seeded scalars with edge values (0, 1, order-1, 8-bit), bases with duplicates and
