@@ -217,7 +217,7 @@ int ref_window(size_t n)
 struct Workspace
 {
     DevBuf aff, isid, digits, counts, offsets, block_sums, sorted, buckets, seg, win, big_list, big_count, entries,
-        cnt_a, off_a, total, chunk_bucket, bnd;
+        cnt_a, off_a, total, chunk_bucket, bnd, parts;
     // second stream for the window combine, which runs behind the bucket work of lower windows
     hipStream_t aux = nullptr;
     static constexpr int MAXQ = 8;
@@ -337,16 +337,21 @@ int scan_exclusive(Workspace &cx, const uint32_t *in, uint32_t *out, uint32_t le
 }
 
 // The plain MSM on device-resident inputs.  d_scalars: n x 8 words; d_points: n x 15 uint64.
+// Point indices in the sort entries have 24 bits: larger MSMs run as chunks (msm_device_any).
+constexpr size_t MSM_MAX_CHUNK = (size_t)1 << 23;
+
+// compat_n: the MSM size that decides the reference's window-8 behaviour (the whole MSM's size
+// when this call is one chunk of it).
 template <class C>
 int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, size_t n, unsigned flags, uint32_t *d_out32,
-               uint64_t *d_outjac, hipStream_t st)
+               uint64_t *d_outjac, hipStream_t st, size_t compat_n = 0)
 {
     if (n >= (1ull << 24))
-        return fail(HS_GPU_EINVAL, "n must be below 2^24 points per MSM");
+        return fail(HS_GPU_EINVAL, "n must be below 2^24 points per chunk");
     Workspace &ws = cx.ws(st);
     const Plan P = make_plan(C::id, n);
     const uint32_t nn = (uint32_t)n;
-    const bool compat = !(flags & HS_GPU_EXACT) && ref_window(n) == 8;
+    const bool compat = !(flags & HS_GPU_EXACT) && ref_window(compat_n ? compat_n : n) == 8;
     // two-level sort geometry: nsb super-buckets of 2^shift buckets per window, blocks of T points
     const int shift = P.c - 1 > 8 ? P.c - 1 - 8 : 0;
     const int nsb = (int)(P.B >> shift);
@@ -599,7 +604,7 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
         if (d.n >= (1u << 24))
         {
             hs_gpu_batch_destroy(b);
-            return fail(HS_GPU_EINVAL, "batch element " + std::to_string(i) + ": n >= 2^24 (use the plain MSM)");
+            return fail(HS_GPU_EINVAL, "batch element " + std::to_string(i) + ": n >= 2^24 (the plain MSM takes larger sizes)");
         }
         TuneEntry te;
         const bool tuned = tune_lookup(1, d.curve, d.n, te);
@@ -750,9 +755,42 @@ int decompress_device(Ctx &cx, const uint32_t *d_in32, size_t n, uint64_t *d_jac
     return 0;
 }
 
+// MSMs of 2^24 points or more: chunks of 2^23 points, each leaving its sum (reference struct) in
+// the stream's workspace, then one kernel adds the chunk sums and encodes the total.
+template <class C>
+int msm_device_chunked(Ctx &cx, const uint32_t *s, const uint64_t *p, size_t n, unsigned flags, uint32_t *o32,
+                       uint64_t *oj, hipStream_t st)
+{
+    Workspace &ws = cx.ws(st);
+    // HS_GPU_CHUNK (points per chunk, for tests of this path at small n) overrides the default
+    const size_t chunk = std::min<size_t>(MSM_MAX_CHUNK, std::max<size_t>(1024, (size_t)env_int("HS_GPU_CHUNK", (int)MSM_MAX_CHUNK)));
+    const size_t k = (n + chunk - 1) / chunk;
+    int r;
+    if ((r = ws.parts.reserve(k * (15 * 8 + 32))))
+        return r;
+    uint64_t *parts = ws.parts.as<uint64_t>();
+    uint32_t *scratch32 = (uint32_t *)(parts + 15 * k);
+    for (size_t j = 0; j < k; j++)
+    {
+        const size_t off = j * chunk, len = std::min(chunk, n - off);
+        if ((r = msm_device<C>(cx, s + 8 * off, p + 15 * off, len, flags, scratch32 + 8 * j, parts + 15 * j, st, n)))
+            return r;
+    }
+    hipLaunchKernelGGL(hs::k_sum_parts<C>, dim3(1), dim3(64), 0, st, parts, (uint32_t)k, o32, oj);
+    HS_CHECK(hipGetLastError());
+    return 0;
+}
+
 int msm_device_any(int curve, Ctx &cx, const void *s, const void *p, size_t n, unsigned flags, void *o32, void *oj,
                    hipStream_t st)
 {
+    const size_t chunk = (size_t)env_int("HS_GPU_CHUNK", 0);
+    if (n >= ((size_t)1 << 24) || (chunk >= 1024 && n > chunk))
+        return curve == HS_GPU_HELIOS
+                   ? msm_device_chunked<hs::HeliosC>(cx, (const uint32_t *)s, (const uint64_t *)p, n, flags,
+                                                     (uint32_t *)o32, (uint64_t *)oj, st)
+                   : msm_device_chunked<hs::SeleneC>(cx, (const uint32_t *)s, (const uint64_t *)p, n, flags,
+                                                     (uint32_t *)o32, (uint64_t *)oj, st);
     if (curve == HS_GPU_HELIOS)
         return msm_device<hs::HeliosC>(cx, (const uint32_t *)s, (const uint64_t *)p, n, flags, (uint32_t *)o32,
                                        (uint64_t *)oj, st);
@@ -1463,7 +1501,7 @@ int hs_gpu_server_submit(hs_gpu_server *s, const hs_gpu_msm_desc *req, int devic
     if ((req->curve != HS_GPU_HELIOS && req->curve != HS_GPU_SELENE) || (req->n && (!req->scalars || !req->points)))
         return fail(HS_GPU_EINVAL, "bad request");
     if (req->n >= ((size_t)1 << 24))
-        return fail(HS_GPU_EINVAL, "request n >= 2^24 (use the plain MSM)");
+        return fail(HS_GPU_EINVAL, "request n >= 2^24 (the plain MSM takes larger sizes)");
     auto *q = new SrvReq();
     q->d = *req;
     q->device_inputs = device_inputs != 0;

@@ -1234,6 +1234,44 @@ __global__ void __launch_bounds__(64) k_finalize_batch(const jac *__restrict__ t
 
 namespace hs
 {
+// Sum of k partial MSM results (reference structs, affine with Z = 1 or the identity 1:1:0) by one
+// quad of lanes, then the encoding: the total of an MSM computed in chunks.
+template <class C>
+__global__ void __launch_bounds__(64) k_sum_parts(const uint64_t *__restrict__ parts, uint32_t k,
+                                                  uint32_t *__restrict__ out32, uint64_t *__restrict__ outjac)
+{
+    using F = typename C::F;
+    if (threadIdx.x >= 4)
+        return;
+    jac acc, t;
+    jac_identity<F>(acc);
+    for (uint32_t j = 0; j < k; j++)
+    {
+        uint64_t l[5];
+        for (int i = 0; i < 5; i++)
+            l[i] = parts[15ull * j + i];
+        fe_from_r51<F>(t.X, l);
+        for (int i = 0; i < 5; i++)
+            l[i] = parts[15ull * j + 5 + i];
+        fe_from_r51<F>(t.Y, l);
+        for (int i = 0; i < 5; i++)
+            l[i] = parts[15ull * j + 10 + i];
+        fe_from_r51<F>(t.Z, l);
+        jac_add_q<F>(acc, acc, t);
+    }
+    if (threadIdx.x != 0)
+        return;
+    aff a;
+    const bool pt = jac_to_aff<F>(a, acc);
+    uint32_t enc[8];
+    encode_aff<F>(enc, a, pt);
+    if (out32)
+        for (int i = 0; i < 8; i++)
+            out32[i] = enc[i];
+    if (outjac)
+        store_r51_aff<F>(outjac, a, pt);
+}
+
 // identity outputs for the empty MSMs (n = 0) of a batch: 0^32 and the reference's identity
 // Jacobian (X = 1, Y = 1, Z = 0 in radix 2^51)
 __global__ void k_identity_out(const BatchDesc *__restrict__ d, uint32_t M)

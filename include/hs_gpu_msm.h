@@ -24,6 +24,9 @@
  * hs_gpu_last_error() describes the most recent failure of the calling thread.
  * Threading: every entry point may be called from several host threads; calls on one device are
  * serialised internally (one in-flight plain MSM per device context).
+ * Sizes: a plain MSM of 2^24 points or more runs as chunks of 2^23 points whose sums are added on
+ * the device (same results); batch elements, server requests and fixed-base sets are limited to
+ * fewer than 2^24 points each.
  */
 #ifndef HS_GPU_MSM_H
 #define HS_GPU_MSM_H

@@ -197,6 +197,23 @@ def _fast_scalars(c, n, seed):
     return raw.tobytes()
 
 
+@pytest.mark.parametrize("c", [0, 1])
+def test_chunked_msm(c, monkeypatch):
+    """MSMs of 2^24 points or more run as chunks whose sums are added on the device; the chunk
+    size is lowered here so the path runs at test sizes.  n = 3000 in chunks of 1024 would hit the
+    reference's window-8 band per chunk but not as a whole: the band follows the whole MSM."""
+    monkeypatch.setenv("HS_GPU_CHUNK", "1024")
+    for n in (3000, 2000, 1025):
+        sc = O.gen_scalars(c, 60 + n, n, edge_frac=0.02)
+        pb = O.gen_sswu_bases(c, 61 + n, n, dup_frac=0.02, id_frac=0.01)
+        pts = GH.to_jac_array(c, pb, n)
+        out, jac = run(c, sc, pts)
+        assert out == O.msm(c, sc, pb, n), n
+        assert GH.jac_to_bytes(c, jac) == out
+        out2, _ = run(c, sc, pts, flags=hs.EXACT)
+        assert out2 == O.msm_naive(c, sc, pb, n), n
+
+
 def test_drop_in_slot_and_n0():
     g = O.generator(0)
     pts = GH.to_jac_array(0, g * 3, 3)
