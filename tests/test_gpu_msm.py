@@ -5,6 +5,7 @@ reference's results on seeded inputs (committed fixture), the oracle at mid size
 full 2^16..2^20 sizes against a closed form (bases on an arithmetic progression of multiples of
 G, so the MSM equals (sum s_i k_i mod order) * G).
 """
+import ctypes
 import random
 
 import numpy as np
@@ -195,6 +196,32 @@ def _fast_scalars(c, n, seed):
         v = (0, 1, O.ORDER[c] - 1, int(raw[i, 0]) | 1)[j % 4]
         raw[i] = np.frombuffer(v.to_bytes(32, "little"), dtype=np.uint8)
     return raw.tobytes()
+
+
+@pytest.mark.parametrize("c,logn", [(0, 16), (1, 16), (1, 18)])
+def test_config2_shape_vs_compiled_reference(c, logn):
+    """The plain-MSM workload shape (SSWU bases with 0.5% duplicates and 0.1% identities, 1% edge
+    scalars) at n = 2^16 and 2^18, against the compiled reference's own MSM bytes on the same
+    inputs (bytes ABI: the points are decoded on the device)."""
+    if O.ref() is None:
+        pytest.skip("compiled reference (oracle/_ref) not built")
+    n = 1 << logn
+    rng = np.random.default_rng(2018 + c)
+    sc = O.gen_scalars(c, 2018 + c, n, edge_frac=0.01)
+    # SSWU images by the compiled reference's own map (fast), then duplicates and identities
+    u = rng.integers(0, 256, size=n * 32, dtype=np.uint8).tobytes()
+    out = ctypes.create_string_buffer(32 * n)
+    getattr(O.ref(), f"hsref_{O.CURVES[c]}_map_to_curve_bytes")(out, u, n)
+    pb = bytearray(out.raw)
+    for j in range(n // 200):
+        i, src = int(rng.integers(1, n)), int(rng.integers(0, n))
+        pb[32 * i:32 * i + 32] = pb[32 * src:32 * src + 32]
+    for j in range(n // 1000):
+        i = int(rng.integers(0, n))
+        pb[32 * i:32 * i + 32] = bytes(32)
+    pb = bytes(pb)
+    want = O.ref_msm(c, sc, pb, n)
+    assert hs.msm_bytes(c, sc, pb) == want
 
 
 @pytest.mark.parametrize("c", [0, 1])
