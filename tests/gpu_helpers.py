@@ -113,3 +113,26 @@ def progression_expected(c, scalars_bytes, n, a, d):
         tot += s * (a + i * d)
     k = tot % order
     return O.scalar_mul(c, k.to_bytes(32, "little"), O.generator(c))
+
+
+def ref_sswu_bases(c, seed, n, dup_frac=0.0, id_frac=0.0):
+    """n SSWU images (compressed) of seeded uniform bytes, mapped by the compiled reference's own
+    map_to_curve, with dup_frac of them copies of another base and id_frac the identity (0^32).
+    None when the compiled reference is not built."""
+    import ctypes as _ct
+
+    r = O.ref()
+    if r is None:
+        return None
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, 256, size=n * 32, dtype=np.uint8).tobytes()
+    out = _ct.create_string_buffer(32 * n)
+    getattr(r, f"hsref_{O.CURVES[c]}_map_to_curve_bytes")(out, u, n)
+    pb = bytearray(out.raw)
+    for _ in range(int(n * dup_frac)):
+        i, src = int(rng.integers(1, n)), int(rng.integers(0, n))
+        pb[32 * i:32 * i + 32] = pb[32 * src:32 * src + 32]
+    for _ in range(int(n * id_frac)):
+        i = int(rng.integers(0, n))
+        pb[32 * i:32 * i + 32] = bytes(32)
+    return bytes(pb)

@@ -60,6 +60,26 @@ def test_batch_exact_mode_window8():
     assert [g[0] for g in got] == want
 
 
+def test_batch_mix_vs_compiled_reference():
+    """The batched-mix shape (sizes log-uniform in [64, 2^14], both curves interleaved, distinct
+    SSWU bases per MSM, uniform scalars), every element against the compiled reference's bytes."""
+    if O.ref() is None:
+        pytest.skip("compiled reference (oracle/_ref) not built")
+    rng = np.random.default_rng(33)
+    items, want = [], []
+    for k in range(24):
+        c = k % 2
+        n = int(np.exp(rng.uniform(np.log(64), np.log(1 << 14))))
+        if k == 5:
+            n = 1700  # the reference's window-8 band
+        sc = O.gen_scalars(c, 3300 + k, n)
+        pb = GH.ref_sswu_bases(c, 3400 + k, n)
+        items.append((c, GH.scalars_np(sc, n), GH.to_jac_array(c, pb, n)))
+        want.append(O.ref_msm(c, sc, pb, n))
+    got = hs.msm_batch(items)
+    assert [g[0] for g in got] == want
+
+
 def test_batch_matches_plain_path():
     items = []
     for k, n in enumerate((100, 4096, 12000)):

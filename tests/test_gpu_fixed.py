@@ -140,3 +140,27 @@ def test_pedersen_batch_4096_generators(c):
         row = np.concatenate([blind[k:k + 1], vals[k]], axis=0)
         want = hs.msm(c, row, pts)[0]
         assert comb[k] == want and pip[k] == want, k
+
+
+@pytest.mark.parametrize("c", [0, 1])
+def test_pedersen_batch_vs_compiled_reference(c):
+    """The Pedersen-batch shape (H + 2^12 SSWU generators held resident, uniform values and
+    blindings) against the compiled reference's helios_pedersen_commit / selene_pedersen_commit."""
+    import ctypes
+
+    if O.ref() is None:
+        pytest.skip("compiled reference (oracle/_ref) not built")
+    m, K = 4096, 2
+    pb = GH.ref_sswu_bases(c, 4400 + c, m + 1)
+    pts = GH.to_jac_array(c, pb, m + 1)
+    rows = [O.gen_scalars(c, 4500 + 10 * c + k, m + 1) for k in range(K)]
+    fs = hs.FixedSet(c, pts)
+    S = np.frombuffer(b"".join(rows), np.uint8).reshape(K, m + 1, 32)
+    comb, _ = fs.pedersen(S[:, 0], S[:, 1:], method=hs.FIXED_COMB)
+    pip, _ = fs.pedersen(S[:, 0], S[:, 1:], method=hs.FIXED_PIPPENGER)
+    fs.close()
+    f = getattr(O.ref(), f"hsref_{O.CURVES[c]}_pedersen_bytes")
+    for k in range(K):
+        out = ctypes.create_string_buffer(32)
+        assert f(out, rows[k][:32], pb[:32], rows[k][32:], pb[32:], m) == 0
+        assert comb[k] == out.raw and pip[k] == out.raw, k

@@ -5,7 +5,6 @@ reference's results on seeded inputs (committed fixture), the oracle at mid size
 full 2^16..2^20 sizes against a closed form (bases on an arithmetic progression of multiples of
 G, so the MSM equals (sum s_i k_i mod order) * G).
 """
-import ctypes
 import random
 
 import numpy as np
@@ -206,20 +205,8 @@ def test_config2_shape_vs_compiled_reference(c, logn):
     if O.ref() is None:
         pytest.skip("compiled reference (oracle/_ref) not built")
     n = 1 << logn
-    rng = np.random.default_rng(2018 + c)
     sc = O.gen_scalars(c, 2018 + c, n, edge_frac=0.01)
-    # SSWU images by the compiled reference's own map (fast), then duplicates and identities
-    u = rng.integers(0, 256, size=n * 32, dtype=np.uint8).tobytes()
-    out = ctypes.create_string_buffer(32 * n)
-    getattr(O.ref(), f"hsref_{O.CURVES[c]}_map_to_curve_bytes")(out, u, n)
-    pb = bytearray(out.raw)
-    for j in range(n // 200):
-        i, src = int(rng.integers(1, n)), int(rng.integers(0, n))
-        pb[32 * i:32 * i + 32] = pb[32 * src:32 * src + 32]
-    for j in range(n // 1000):
-        i = int(rng.integers(0, n))
-        pb[32 * i:32 * i + 32] = bytes(32)
-    pb = bytes(pb)
+    pb = GH.ref_sswu_bases(c, 2018 + c, n, dup_frac=0.005, id_frac=0.001)
     want = O.ref_msm(c, sc, pb, n)
     assert hs.msm_bytes(c, sc, pb) == want
 
