@@ -371,6 +371,55 @@ def cpu_baseline(hs_pts_cpu, sc_cpu, n_sample):
     }
 
 
+def cpu_many(hs_pts_cpu, sc_cpu, jobs, threads):
+    """The reference CPU MSM on many independent MSMs at once: for each (curve, n) in `jobs`,
+    `threads` host threads each run one whole reference MSM of that size (the function is
+    reentrant), on the first n bases/scalars of the plain workload.  Returns points/s."""
+    lib = os.path.join(ROOT, "oracle", "_ref", "libhsref.so")
+    if not os.path.exists(lib) or not jobs:
+        return None
+    L = ctypes.CDLL(lib)
+    L.hsref_init.restype = ctypes.c_uint
+    L.hsref_init()
+    pts, tot = 0, 0.0
+    for curve, n in jobs:
+        f = getattr(L, ("hsref_helios_time_msm", "hsref_selene_time_msm")[curve])
+        f.restype = ctypes.c_double
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+        s = sc_cpu[curve][:n].copy()
+        p = hs_pts_cpu[curve][:n].copy()
+        tot += f(s.ctypes.data, p.ctypes.data, n, threads, threads)
+        pts += threads * n
+    return pts / tot, tot
+
+
+def add_cpu_legs(rec, pts_cpu, sc_cpu, args):
+    """N-thread reference baselines for the batched mix, the Pedersen batch and the served stream,
+    each on a bounded sample of its own size distribution."""
+    import numpy as np
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rng = np.random.default_rng(args.seed + 5000)
+    legs = {
+        "batched_mix": [(k % 2, int(np.exp(rng.uniform(np.log(64), np.log(1 << 14))))) for k in range(24)],
+        "pedersen_batch": [(0, 4097), (1, 4097)],
+        "serving": [(int(rng.integers(0, 2)), int(np.exp(rng.uniform(np.log(32), np.log(1 << 18)))))
+                    for _ in range(8)],
+    }
+    for leg, jobs in legs.items():
+        if not rec.get(leg):
+            continue
+        r = cpu_many(pts_cpu, sc_cpu, jobs, threads)
+        if r is None:
+            continue
+        rate, secs = r
+        rec[leg]["cpu_baseline"] = {
+            "value": rate, "unit": "points/s", "cores": threads, "kind": "reference",
+            "sample": (f"{len(jobs)} MSM sizes drawn from this leg's distribution; for each, {threads} host threads "
+                       f"run one whole reference MSM each at the same time ({secs:.1f} s in total)"),
+        }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -537,6 +586,7 @@ def main():
     rec["host_slot_points_per_s"] = 2 * n / t_host
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(pts_cpu, sc_cpu, min(n, args.cpu_sample))
+        add_cpu_legs(rec, pts_cpu, sc_cpu, args)
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)
