@@ -75,20 +75,21 @@ __device__ __forceinline__ void fe_mul_q2(fe &r0, fe &r1, const fe &a0, const fe
 // (X - delta)(X + delta), gamma^2}, {alpha^2}, {alpha (4 beta - X3)}.
 template <class F> __device__ __forceinline__ void jac_dbl_q(jac &p)
 {
+    // Operand bounds: Y + Z and X + delta stay un-normalised (limbs <= 2^30), which fe_mul accepts
+    // for both operands; 4 beta - X3 is a lazy subtraction, multiplied by the normalised alpha.
     fe yz, delta, gamma, yz2, xm, xp, beta, t, g2, alpha, X3, Y3, Z3, t0;
-    fe_add<F>(yz, p.Y, p.Z);
+    fe_add_lazy<F>(yz, p.Y, p.Z);
     fe_mul_q3<F>(delta, gamma, yz2, p.Z, p.Z, p.Y, p.Y, yz, yz);
     fe_sub<F>(xm, p.X, delta);
-    fe_add<F>(xp, p.X, delta);
+    fe_add_lazy<F>(xp, p.X, delta);
     fe_mul_q3<F>(beta, t, g2, p.X, gamma, xm, xp, gamma, gamma);
     fe_mul_small<F>(alpha, t, 3);
     fe_sq<F>(X3, alpha);
     fe_mul_small<F>(t0, beta, 8);
     fe_sub<F>(X3, X3, t0);
-    fe_sub<F>(Z3, yz2, gamma);
-    fe_sub<F>(Z3, Z3, delta);
+    fe_sub2<F>(Z3, yz2, gamma, delta);
     fe_mul_small<F>(t0, beta, 4);
-    fe_sub<F>(t0, t0, X3);
+    fe_sub_lazy<F>(t0, t0, X3);
     fe_mul<F>(Y3, alpha, t0);
     fe_mul_small<F>(t0, g2, 8);
     fe_sub<F>(Y3, Y3, t0);
@@ -121,13 +122,12 @@ template <class F> __device__ __forceinline__ void jac_add_q(jac &r, const jac &
         return;
     }
     fe Z1Z1, Z2Z2, zs, zs2, U1, U2, a, b, S1, S2, H, rr, I, Zt, Z3, J, V, r2, X3, Y3, t0, t1;
-    fe_add<F>(zs, p.Z, q.Z);
+    fe_add_lazy<F>(zs, p.Z, q.Z); // squared below: limbs <= 2^30 are fine for both operands
     fe_mul_q3<F>(Z1Z1, Z2Z2, zs2, p.Z, p.Z, q.Z, q.Z, zs, zs);
     fe_mul_q4<F>(U1, U2, a, b, p.X, Z2Z2, q.X, Z1Z1, q.Z, Z2Z2, p.Z, Z1Z1);
     fe_sub<F>(H, U2, U1);
-    fe_add<F>(t0, H, H);
-    fe_sub<F>(Zt, zs2, Z1Z1);
-    fe_sub<F>(Zt, Zt, Z2Z2);
+    fe_add_lazy<F>(t0, H, H);
+    fe_sub2<F>(Zt, zs2, Z1Z1, Z2Z2);
     fe_mul_q4<F>(S1, S2, I, Z3, p.Y, a, q.Y, b, t0, t0, Zt, H);
     fe_sub<F>(rr, S2, S1);
     if (fe_is_zero<F>(H))
@@ -143,13 +143,10 @@ template <class F> __device__ __forceinline__ void jac_add_q(jac &r, const jac &
     }
     fe_add<F>(rr, rr, rr);
     fe_mul_q3<F>(J, V, r2, H, I, U1, I, rr, rr);
-    fe_sub<F>(X3, r2, J);
-    fe_add<F>(t0, V, V);
-    fe_sub<F>(X3, X3, t0);
-    fe_sub<F>(t0, V, X3);
+    fe_sub3<F>(X3, r2, J, V, V);  // rr^2 - J - 2V
+    fe_sub_lazy<F>(t0, V, X3);    // multiplied by the normalised rr only
     fe_mul_q2<F>(t1, Y3, rr, t0, S1, J);
-    fe_add<F>(t0, Y3, Y3);
-    fe_sub<F>(Y3, t1, t0);
+    fe_sub2<F>(Y3, t1, Y3, Y3);   // rr (V - X3) - 2 S1 J
     r.X = X3;
     r.Y = Y3;
     r.Z = Z3;
