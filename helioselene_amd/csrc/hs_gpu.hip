@@ -582,7 +582,38 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
     auto *b = new hs_gpu_batch();
     b->flags = flags;
     b->in_arena = arena != nullptr;
-    // group by (curve, window width)
+    // group by (curve, window width).  Every group is one chain of kernels whose latency (the
+    // combine's 256 doublings above all) does not shrink with the group's size, so window widths
+    // are merged into at most HS_GPU_BATCH_GROUPS (default 2) groups per curve: each width maps
+    // to the largest width of its share of the curve's sorted distinct widths.  Batches planned
+    // inside a serving arena keep their own widths (the admission footprint assumes them).
+    std::vector<int> want_c(count, 0);
+    for (size_t i = 0; i < count; i++)
+    {
+        if (descs[i].n == 0 || descs[i].n >= (1u << 24) ||
+            (descs[i].curve != HS_GPU_HELIOS && descs[i].curve != HS_GPU_SELENE))
+            continue;
+        TuneEntry te;
+        const int c = tune_lookup(1, descs[i].curve, descs[i].n, te) ? te.c : choose_c_batch(descs[i].n);
+        want_c[i] = c < 4 ? 4 : c > 13 ? 13 : c;
+    }
+    const int max_groups = arena ? 64 : std::max(1, env_int("HS_GPU_BATCH_GROUPS", 2));
+    int cmap[2][14] = {};
+    for (int cv = 0; cv < 2; cv++)
+    {
+        std::vector<int> cs;
+        for (size_t i = 0; i < count; i++)
+            if (want_c[i] && descs[i].curve == cv && std::find(cs.begin(), cs.end(), want_c[i]) == cs.end())
+                cs.push_back(want_c[i]);
+        std::sort(cs.begin(), cs.end());
+        const size_t k = std::min(cs.size(), (size_t)max_groups);
+        for (size_t j = 0; j < cs.size(); j++)
+        {
+            const size_t part = j * k / cs.size();              // which of the k shares
+            const size_t last = ((part + 1) * cs.size() + k - 1) / k - 1; // its largest member
+            cmap[cv][cs[j]] = cs[std::min(last, cs.size() - 1)];
+        }
+    }
     std::vector<std::pair<int, int>> keys;
     std::vector<hs::BatchDesc> empties;
     for (size_t i = 0; i < count; i++)
@@ -608,8 +639,7 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
         }
         TuneEntry te;
         const bool tuned = tune_lookup(1, d.curve, d.n, te);
-        int c = tuned ? te.c : choose_c_batch(d.n);
-        c = c < 4 ? 4 : c > 13 ? 13 : c;
+        const int c = cmap[d.curve][want_c[i]];
         BatchGroup *g = nullptr;
         for (size_t k = 0; k < keys.size(); k++)
             if (keys[k].first == d.curve && keys[k].second == c)
