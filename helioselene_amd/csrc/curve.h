@@ -61,7 +61,7 @@ template <class F> HS_HD void tangent_m(fe &m, const fe &x, const fe &z)
 // dbl-2008-s-1 with a = -3 (M = 3X^2 - 3ZZ^2 = 3(X-ZZ)(X+ZZ)); 7M + 2S
 template <class F> HS_HD void xyzz_dbl(xyzz &r, const xyzz &p)
 {
-    fe U, V, W, S, M, t;
+    fe U, V, W, S, M, t, nY;
     fe_add<F>(U, p.Y, p.Y);
     fe_sq<F>(V, U);
     fe_mul<F>(W, U, V);
@@ -70,10 +70,9 @@ template <class F> HS_HD void xyzz_dbl(xyzz &r, const xyzz &p)
     fe X3, Y3;
     fe_sq<F>(X3, M);
     fe_sub2<F>(X3, X3, S, S); // M^2 - 2S
-    fe_sub_lazy<F>(t, S, X3); // only multiplied by the normalised M
-    fe_mul<F>(Y3, M, t);
-    fe_mul<F>(t, W, p.Y);
-    fe_sub<F>(Y3, Y3, t);
+    fe_sub_lazy2<F>(t, S, X3);
+    fe_neg_lazy2<F>(nY, p.Y);
+    fe_mul_sum2<F>(Y3, M, t, nY, W); // M (S - X3) - W Y
     fe_mul<F>(r.ZZ, V, p.ZZ);
     fe_mul<F>(r.ZZZ, W, p.ZZZ);
     r.X = X3;
@@ -83,7 +82,7 @@ template <class F> HS_HD void xyzz_dbl(xyzz &r, const xyzz &p)
 // mdbl-2008-s-1 with a = -3: doubling of an affine point into xyzz
 template <class F> HS_HD void xyzz_mdbl(xyzz &r, const aff &p)
 {
-    fe one, U, V, W, S, M, t;
+    fe one, U, V, W, S, M, t, nY;
     fe_one(one);
     fe_add<F>(U, p.y, p.y);
     fe_sq<F>(V, U);
@@ -93,10 +92,9 @@ template <class F> HS_HD void xyzz_mdbl(xyzz &r, const aff &p)
     fe X3, Y3;
     fe_sq<F>(X3, M);
     fe_sub2<F>(X3, X3, S, S); // M^2 - 2S
-    fe_sub_lazy<F>(t, S, X3); // only multiplied by the normalised M
-    fe_mul<F>(Y3, M, t);
-    fe_mul<F>(t, W, p.y);
-    fe_sub<F>(Y3, Y3, t);
+    fe_sub_lazy2<F>(t, S, X3);
+    fe_neg_lazy2<F>(nY, p.y);
+    fe_mul_sum2<F>(Y3, M, t, nY, W); // M (S - X3) - W y
     r.X = X3;
     r.Y = Y3;
     r.ZZ = V;
@@ -126,17 +124,16 @@ template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
             acc_id = true; // acc == -q
         return;
     }
-    fe PP, PPP, Q, t;
+    fe PP, PPP, Q, t, nY;
     fe_sq<F>(PP, P);
     fe_mul<F>(PPP, P, PP);
     fe_mul<F>(Q, acc.X, PP);
     fe X3, Y3;
     fe_sq<F>(X3, R);
     fe_sub3<F>(X3, X3, PPP, Q, Q); // R^2 - PPP - 2Q
-    fe_sub_lazy<F>(t, Q, X3);      // only multiplied by the normalised R
-    fe_mul<F>(Y3, R, t);
-    fe_mul<F>(t, acc.Y, PPP);
-    fe_sub<F>(Y3, Y3, t);
+    fe_sub_lazy2<F>(t, Q, X3);
+    fe_neg_lazy2<F>(nY, acc.Y);
+    fe_mul_sum2<F>(Y3, R, t, nY, PPP); // R (Q - X3) - Y1 PPP
     fe_mul<F>(acc.ZZ, acc.ZZ, PP);
     fe_mul<F>(acc.ZZZ, acc.ZZZ, PPP);
     acc.X = X3;
@@ -171,17 +168,16 @@ template <class F> HS_HD void xyzz_add(xyzz &r, const xyzz &p, const xyzz &q)
             xyzz_identity<F>(r);
         return;
     }
-    fe PP, PPP, Q, t;
+    fe PP, PPP, Q, t, nY;
     fe_sq<F>(PP, P);
     fe_mul<F>(PPP, P, PP);
     fe_mul<F>(Q, U1, PP);
     fe X3, Y3;
     fe_sq<F>(X3, R);
     fe_sub3<F>(X3, X3, PPP, Q, Q); // R^2 - PPP - 2Q
-    fe_sub_lazy<F>(t, Q, X3);      // only multiplied by the normalised R
-    fe_mul<F>(Y3, R, t);
-    fe_mul<F>(t, S1, PPP);
-    fe_sub<F>(Y3, Y3, t);
+    fe_sub_lazy2<F>(t, Q, X3);
+    fe_neg_lazy2<F>(nY, S1);
+    fe_mul_sum2<F>(Y3, R, t, nY, PPP); // R (Q - X3) - S1 PPP
     fe ZZ3, ZZZ3;
     fe_mul<F>(ZZ3, p.ZZ, q.ZZ);
     fe_mul<F>(ZZ3, ZZ3, PP);

@@ -47,6 +47,8 @@ struct Fp
     }
     // 257 p with every limb in [2^31, 2^31 + 2^29): bias of a subtraction of three normalised values
     static HS_HD uint32_t bias3(int i) { return i == 0 ? 0x9fffecedu : i == 8 ? 0x807ffffbu : 0x9ffffffbu; }
+    // 65 p with every limb in [2^29, 2^30): bias of the small-limb lazy forms
+    static HS_HD uint32_t bias2(int i) { return i == 0 ? 0x3ffffb2du : i == 8 ? 0x207ffffeu : 0x3ffffffeu; }
     // modulus and 2^255 mod m as 32-bit words (canonicalisation)
     static HS_HD uint32_t mw(int i) { return i == 0 ? 0xffffffedu : i == 7 ? 0x7fffffffu : 0xffffffffu; }
     static HS_HD uint32_t cw(int i) { return i == 0 ? 19u : 0u; }
@@ -119,6 +121,20 @@ struct Fq
         case 4: return 0x9ffbf3ebu;
         case 8: return 0x807ffffbu;
         default: return 0x9ffffffbu;
+        }
+    }
+    // 65 q with every limb in [2^29, 2^30)
+    static HS_HD uint32_t bias2(int i)
+    {
+        switch (i)
+        {
+        case 0: return 0x2319af5fu;
+        case 1: return 0x235b7885u;
+        case 2: return 0x30105d06u;
+        case 3: return 0x3b06b520u;
+        case 4: return 0x3ffef9f4u;
+        case 8: return 0x207ffffeu;
+        default: return 0x3ffffffeu;
         }
     }
     // q = 2^255 - gamma, gamma = 0x408087d3489a94a791492d8d86d83861 (reference fq51.h:39-62)
@@ -270,6 +286,22 @@ template <class F> HS_HD void fe_add_lazy(fe &r, const fe &a, const fe &b)
         r.v[i] = a.v[i] + b.v[i];
 }
 
+// Small-limb lazy forms for fe_mul_sum2: r = a - b (limbs < 2^29 + 2^30) and r = -a (limbs
+// < 2^30) for normalised a, b, via the bias 65 m whose limbs lie in [2^29, 2^30).
+template <class F> HS_HD void fe_sub_lazy2(fe &r, const fe &a, const fe &b)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = a.v[i] + F::bias2(i) - b.v[i];
+}
+
+template <class F> HS_HD void fe_neg_lazy2(fe &r, const fe &a)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = F::bias2(i) - a.v[i];
+}
+
 template <class F> HS_HD void fe_neg(fe &r, const fe &a)
 {
     fe z;
@@ -404,6 +436,26 @@ template <class F> HS_HD void fe_mul(fe &r, const fe &a, const fe &b)
         for (int j = 0; j < NL; j++)
             c[i + j] = mad64(a.v[i], b.v[j], c[i + j]);
     reduce_cols<F>(r, c);
+}
+
+// r = a b + c d with one reduction.  Operands: a and d normalised, b from fe_sub_lazy2 and c from
+// fe_neg_lazy2 (or normalised): columns stay below 9 * 2^58 * (3 + 2) < 2^63.5, inside the
+// bounds of reduce_cols.
+template <class F> HS_HD void fe_mul_sum2(fe &r, const fe &a, const fe &b, const fe &c, const fe &d)
+{
+    uint64_t w[2 * NL - 1];
+#pragma unroll
+    for (int k = 0; k < 2 * NL - 1; k++)
+        w[k] = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+#pragma unroll
+        for (int j = 0; j < NL; j++)
+        {
+            w[i + j] = mad64(a.v[i], b.v[j], w[i + j]);
+            w[i + j] = mad64(c.v[i], d.v[j], w[i + j]);
+        }
+    reduce_cols<F>(r, w);
 }
 
 template <class F> HS_HD void fe_sq(fe &r, const fe &a)

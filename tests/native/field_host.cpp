@@ -132,6 +132,43 @@ extern "C" int hst_fe_op_limbs(int field, int o, uint8_t *out40, const uint32_t 
     return 0;
 }
 
+// fe_mul_sum2(a, b, c, d) on raw limbs (out40 as hst_fe_op_limbs); with lazy != 0 the operands
+// are formed as in the point formulas: b = fe_sub_lazy2(lb, lb2), c = fe_neg_lazy2(lc)
+template <class F>
+static void mul_sum2_limbs(uint8_t *out40, const uint32_t *la, const uint32_t *lb, const uint32_t *lb2,
+                           const uint32_t *lc, const uint32_t *ld, int lazy)
+{
+    fe a, b, b2, c, d, r;
+    for (int i = 0; i < NL; i++)
+    {
+        a.v[i] = la[i];
+        b.v[i] = lb[i];
+        b2.v[i] = lb2[i];
+        c.v[i] = lc[i];
+        d.v[i] = ld[i];
+    }
+    if (lazy)
+    {
+        fe_sub_lazy2<F>(b, b, b2);
+        fe_neg_lazy2<F>(c, c);
+    }
+    fe_mul_sum2<F>(r, a, b, c, d);
+    st_raw(out40, r);
+    uint32_t mx = 0;
+    for (int i = 0; i < NL; i++)
+        mx = r.v[i] > mx ? r.v[i] : mx;
+    out40[39] = mx > (1u << 29) ? 1 : 0;
+}
+
+extern "C" void hst_fe_mul_sum2(int field, uint8_t *out40, const uint32_t *la, const uint32_t *lb, const uint32_t *lb2,
+                                const uint32_t *lc, const uint32_t *ld, int lazy)
+{
+    if (field == 0)
+        mul_sum2_limbs<Fp>(out40, la, lb, lb2, lc, ld, lazy);
+    else
+        mul_sum2_limbs<Fq>(out40, la, lb, lb2, lc, ld, lazy);
+}
+
 extern "C" void hst_fe_op(int field, int o, uint8_t *out, const uint8_t *a, const uint8_t *b)
 {
     g_field = field;

@@ -258,6 +258,35 @@ def test_field_extreme_limbs(L, k):
         assert L.hst_fe_op_limbs(k, 4, out, (ctypes.c_uint32 * 9)(*d), (ctypes.c_uint32 * 9)()) == 0, hex(v)
 
 
+@pytest.mark.parametrize("k", [0, 1])
+def test_mul_sum2_extreme_limbs(L, k):
+    """a b + c d with one reduction, as the point formulas form it: b = x - y with the small-limb
+    bias, c = -z, every input at the representation's limits (limbs 0, 2^29 - 1 or 2^29); also
+    with the lazy operands passed raw at their largest limbs (2^29 + 2^30 - 2 and 2^30 - 2)."""
+    m = O.FIELD[k]
+    rnd = random.Random(77 + k)
+    top = 1 << 29
+    pats = [[top] * 9, [top - 1] * 9, [0] * 9, [top, 0] * 4 + [top], [0, top] * 4 + [0]]
+    pats += [[rnd.choice([0, top, top - 1, rnd.getrandbits(29)]) for _ in range(9)] for _ in range(300)]
+    out = ctypes.create_string_buffer(40)
+    U = lambda l: (ctypes.c_uint32 * 9)(*l)
+    for i, la in enumerate(pats):
+        x, y, z, d = (pats[(i * s + 1) % len(pats)] for s in (3, 5, 7, 11))
+        A, X, Y, Z, D = (_limbs_value(v) for v in (la, x, y, z, d))
+        L.hst_fe_mul_sum2(k, out, U(la), U(x), U(y), U(z), U(d), 1)
+        assert out.raw[39] == 0, ("limb bound", i)
+        v = int.from_bytes(out.raw[:39], "little")
+        assert v < (1 << 262) and v % m == (A * (X - Y) - Z * D) % m, i
+    hi_b, hi_c = top + (1 << 30) - 2, (1 << 30) - 2
+    for b, c in (([hi_b] * 9, [hi_c] * 9), ([hi_b, 0] * 4 + [hi_b], [0, hi_c] * 4 + [0])):
+        for a, d in (([top] * 9, [top] * 9), ([top - 1] * 9, [top] * 9)):
+            L.hst_fe_mul_sum2(k, out, U(a), U(b), U([0] * 9), U(c), U(d), 0)
+            assert out.raw[39] == 0
+            v = int.from_bytes(out.raw[:39], "little")
+            want = _limbs_value(a) * _limbs_value(b) + _limbs_value(c) * _limbs_value(d)
+            assert v < (1 << 262) and v % m == want % m
+
+
 @pytest.mark.parametrize("c", [0, 1])
 def test_jacobian_chain_vs_oracle(L, c):
     """xyzz -> Jacobian conversion, dbl-2001-b chains and add-2007-bl (including P + P and
