@@ -395,15 +395,15 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     cx.mark(st, -1);
     hipLaunchKernelGGL(hs::k_ingest<C>, grd, blk, 0, st, d_points, nn, aff, isid);
     cx.mark(st, 0);
-    hipLaunchKernelGGL(hs::k_recode16<C>, grd, blk, 0, st, d_scalars, isid, nn, P.c, P.W, compat ? 1 : 0, digits);
+    hipLaunchKernelGGL(hs::k_recode_count<C>, dim3(nblk), dim3(hs::SORT_COARSE_THREADS), lds, st, d_scalars, isid, nn,
+                       P.c, P.W, compat ? 1 : 0, nsb, shift, T, nblk, digits, cnt_a);
     cx.mark(st, 1);
-    hipLaunchKernelGGL(hs::k_coarse_count, dim3(nblk), blk, lds, st, digits, nn, P.W, nsb, shift, T, nblk, cnt_a);
     hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, cnt_a + (size_t)nh * nblk, 1u, 0u);
     if ((r = scan_exclusive(ws, cnt_a, off_a, nh * nblk + 1, st)))
         return r;
-    hipLaunchKernelGGL(hs::k_coarse_scatter, dim3(nblk), blk, lds, st, digits, nn, P.W, nsb, shift, T, nblk, off_a,
+    hipLaunchKernelGGL(hs::k_coarse_scatter, dim3(nblk), dim3(hs::SORT_COARSE_THREADS), lds, st, digits, nn, P.W, nsb, shift, T, nblk, off_a,
                        entries);
-    hipLaunchKernelGGL(hs::k_fine_sort, dim3(nh), blk, 0, st, entries, off_a, nblk, nsb, shift, P.B, nh, sorted,
+    hipLaunchKernelGGL(hs::k_fine_sort, dim3(nh), dim3(hs::SORT_FINE_THREADS), 0, st, entries, off_a, nblk, nsb, shift, P.B, nh, sorted,
                        offsets);
     cx.mark(st, 2);
     // Windows are processed from the top in Q groups; group q's combine (Horner over its windows)

@@ -294,50 +294,50 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ di
 // digits lie in [-(2^(c-1)) + 1, 2^(c-1)] with c <= 16).
 constexpr uint16_t DIGIT_ZERO = 0xffffu;
 
-template <class C>
-__global__ void __launch_bounds__(256) k_recode16(const uint32_t *__restrict__ scalars, const uint8_t *__restrict__ is_id,
-                                                  uint32_t n, int c, int W, int w8_compat, uint16_t *__restrict__ digits)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    uint32_t s[8];
-    const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * i);
-    uint4 a = sp[0], b = sp[1];
-    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
-    s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
-    uint32_t k[8];
-    scalar_prepare<typename C::O>(k, s, w8_compat != 0);
-    const bool skip = is_id[i] != 0;
-    uint32_t carry = 0;
-    for (int w = 0; w < W; w++)
-    {
-        uint32_t d = recode_digit(k, c, w, carry);
-        uint16_t code = (d == 0 || skip) ? DIGIT_ZERO : (uint16_t)(((d & 0x7fffffffu) - 1) | ((d >> 31) << 15));
-        digits[(size_t)w * n + i] = code;
-    }
-}
+// Workgroup sizes of the sort passes.  The LDS-atomic loops are latency-bound and want many waves
+// per workgroup; 512 threads measured best with another MSM's bucket kernel sharing the CUs
+// (1024-thread workgroups are faster alone but wait for whole free CUs).
+#ifndef HS_SORT_COARSE_THREADS
+#define HS_SORT_COARSE_THREADS 512
+#endif
+#ifndef HS_SORT_FINE_THREADS
+#define HS_SORT_FINE_THREADS 512
+#endif
+constexpr int SORT_COARSE_THREADS = HS_SORT_COARSE_THREADS;
+constexpr int SORT_FINE_THREADS = HS_SORT_FINE_THREADS;
 
-// Coarse pass: block b takes points [b*T, b*T + T) of every window and counts them per
-// (window, super-bucket), super-bucket = bucket >> shift.  counts[(w*nsb + sb) * nblk + b].
-__global__ void __launch_bounds__(256) k_coarse_count(const uint16_t *__restrict__ digits, uint32_t n, int W,
-                                                      int nsb, int shift, uint32_t T, uint32_t nblk,
-                                                      uint32_t *__restrict__ counts)
+// Recode fused with the coarse count.  Block b recodes the points [b*T, b*T + T): the digit of
+// window w of point i goes to digits[w * n + i], and the block counts its non-zero digits per
+// (window, super-bucket), super-bucket = bucket >> shift, into counts[(w*nsb + sb) * nblk + b].
+template <class C>
+__global__ void __launch_bounds__(SORT_COARSE_THREADS) k_recode_count(
+    const uint32_t *__restrict__ scalars, const uint8_t *__restrict__ is_id, uint32_t n, int c, int W, int w8_compat,
+    int nsb, int shift, uint32_t T, uint32_t nblk, uint16_t *__restrict__ digits, uint32_t *__restrict__ counts)
 {
     extern __shared__ uint32_t hist[];
-    const uint32_t b = blockIdx.x, beg = b * T, cnt = min(T, n - beg), nh = (uint32_t)W * nsb;
+    const uint32_t b = blockIdx.x, beg = b * T, end = min(beg + T, n), nh = (uint32_t)W * nsb;
     for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
         hist[j] = 0;
     __syncthreads();
-    for (int w = 0; w < W; w++)
+    for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x)
     {
-        const uint16_t *dw = digits + (size_t)w * n + beg;
-        uint32_t *hw = hist + (uint32_t)w * nsb;
-        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x)
+        uint32_t s[8];
+        const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * i);
+        const uint4 a = sp[0], bb = sp[1];
+        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+        s[4] = bb.x; s[5] = bb.y; s[6] = bb.z; s[7] = bb.w;
+        uint32_t k[8];
+        scalar_prepare<typename C::O>(k, s, w8_compat != 0);
+        const bool skip = is_id[i] != 0;
+        uint32_t carry = 0;
+        for (int w = 0; w < W; w++)
         {
-            const uint16_t code = dw[i];
-            if (code != DIGIT_ZERO)
-                atomicAdd(&hw[(code & 0x7fffu) >> shift], 1u);
+            const uint32_t d = recode_digit(k, c, w, carry);
+            const uint32_t mag1 = (d & 0x7fffffffu) - 1;
+            const bool zero = d == 0 || skip;
+            digits[(size_t)w * n + i] = zero ? DIGIT_ZERO : (uint16_t)(mag1 | ((d >> 31) << 15));
+            if (!zero)
+                atomicAdd(&hist[(uint32_t)w * nsb + (mag1 >> shift)], 1u);
         }
     }
     __syncthreads();
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(256) k_coarse_count(const uint16_t *__restrict
 }
 
 // Coarse scatter: entry = point index (bits 0..23) | sub-bucket << 24 | sign << 31
-__global__ void __launch_bounds__(256) k_coarse_scatter(const uint16_t *__restrict__ digits, uint32_t n, int W,
+__global__ void __launch_bounds__(SORT_COARSE_THREADS) k_coarse_scatter(const uint16_t *__restrict__ digits, uint32_t n, int W,
                                                         int nsb, int shift, uint32_t T, uint32_t nblk,
                                                         const uint32_t *__restrict__ offs,
                                                         uint32_t *__restrict__ entries)
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(256) k_coarse_scatter(const uint16_t *__restri
 
 // Fine pass: one block per (window, super-bucket) region; orders the region by bucket, writes
 // the final entries (point index | sign << 31) and the bucket offsets of the region.
-__global__ void __launch_bounds__(256) k_fine_sort(const uint32_t *__restrict__ entries, const uint32_t *__restrict__ offs,
+__global__ void __launch_bounds__(SORT_FINE_THREADS) k_fine_sort(const uint32_t *__restrict__ entries, const uint32_t *__restrict__ offs,
                                                    uint32_t nblk, int nsb, int shift, uint32_t B, uint32_t nregions,
                                                    uint32_t *__restrict__ sorted, uint32_t *__restrict__ offsets)
 {
