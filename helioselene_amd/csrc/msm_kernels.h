@@ -295,10 +295,10 @@ __global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ di
 constexpr uint16_t DIGIT_ZERO = 0xffffu;
 
 // Workgroup sizes of the sort passes.  The LDS-atomic loops are latency-bound and want many waves
-// per workgroup; 512 threads measured best with another MSM's bucket kernel sharing the CUs
-// (1024-thread workgroups are faster alone but wait for whole free CUs).
+// per workgroup: 1024-thread coarse workgroups (one per CU) measured fastest alone and with two
+// MSMs per curve in flight; with exactly one MSM per curve in flight 512 threads did better.
 #ifndef HS_SORT_COARSE_THREADS
-#define HS_SORT_COARSE_THREADS 512
+#define HS_SORT_COARSE_THREADS 1024
 #endif
 #ifndef HS_SORT_FINE_THREADS
 #define HS_SORT_FINE_THREADS 512
