@@ -387,7 +387,18 @@ template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
             for (int j = 0; j < F::NR; j++)
                 if (i + j < NL)
                     w[i + j] = mad64(m[i], F::r(j), w[i + j]);
-        // columns 0..8 (< 2^62.3) -> digits; the carry out of the top (< 2^34) times R
+        // Column 8's bits from 2^261 up (top < 2^35) fold back as top * R = tlo R + thi R 2^29
+        // into columns 0..5 before the carry pass (every column stays below 56 * 2^58 < 2^64).
+        // The carries into limb 8 are then < 2^35, so what leaves limb 8 is < 2^7.
+        const uint64_t top = w[NL - 1] >> 29;
+        w[NL - 1] &= M29;
+        const uint32_t tlo = (uint32_t)top & M29, thi = (uint32_t)(top >> 29);
+#pragma unroll
+        for (int j = 0; j < F::NR; j++)
+        {
+            w[j] = mad64(tlo, F::r(j), w[j]);
+            w[j + 1] = mad64(thi, F::r(j), w[j + 1]);
+        }
         t = 0;
 #pragma unroll
         for (int k = 0; k < NL; k++)
@@ -395,32 +406,7 @@ template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
             t = w[k] + (t >> 29);
             r.v[k] = (uint32_t)t & M29;
         }
-        const uint64_t top = t >> 29;
-        const uint32_t tlo = (uint32_t)top & M29, thi = (uint32_t)(top >> 29);
-        // top * R = tlo * R + thi * R * 2^29: at most 6 columns, carried once more
-        uint64_t u = 0;
-#pragma unroll
-        for (int k = 0; k < F::NR + 1; k++)
-        {
-            uint64_t x = (uint64_t)r.v[k] + (u >> 29);
-            if (k < F::NR)
-                x = mad64(tlo, F::r(k), x);
-            if (k >= 1)
-                x = mad64(thi, F::r(k - 1), x);
-            u = x;
-            r.v[k] = (uint32_t)u & M29;
-        }
-        // remaining carry (< 2^5) ripples into limb 6 at most once past a full limb
-        uint32_t cc = (uint32_t)(u >> 29);
-#pragma unroll
-        for (int k = F::NR + 1; k < NL; k++)
-        {
-            const uint32_t y = r.v[k] + cc;
-            r.v[k] = y & M29;
-            cc = y >> 29;
-        }
-        // cc (0 or 1) is 2^261 = R once more: limbs here are small enough to absorb it directly
-        fold_top<F>(r, cc);
+        fold_top<F>(r, (uint32_t)(t >> 29));
     }
 }
 
