@@ -423,8 +423,9 @@ def add_cpu_legs(rec, pts_cpu, sc_cpu, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=40,
+                    help="timed steps (enough that filling and draining the stream pipeline is a small part)")
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
