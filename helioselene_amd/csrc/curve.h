@@ -102,12 +102,15 @@ template <class F> HS_HD void xyzz_mdbl(xyzz &r, const aff &p)
 }
 
 // acc += q (affine), madd-2008-s, with identity / doubling / inverse handling.
-// `acc_id` carries acc's identity state so the hot loop needs no zero test on ZZ.
+// `acc_id` carries acc's identity state so the hot loop needs no zero test on ZZ.  q.x is
+// normalised; q.y may also be a lazy negation (fe_neg_lazy2, limbs < 2^30): on the common path it
+// is only a product operand, and the identity / doubling paths normalise it first.
 template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
 {
     if (acc_id)
     {
         xyzz_from_aff<F>(acc, q);
+        fe_norm<F>(acc.Y);
         acc_id = false;
         return;
     }
@@ -116,10 +119,14 @@ template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
     fe_mul<F>(S2, q.y, acc.ZZZ);
     fe_sub<F>(P, U2, acc.X);
     fe_sub<F>(R, S2, acc.Y);
-    if (fe_is_zero<F>(P))
+    if (fe_is_zero_norm<F>(P))
     {
-        if (fe_is_zero<F>(R))
-            xyzz_mdbl<F>(acc, q); // acc == q
+        if (fe_is_zero_norm<F>(R))
+        {
+            aff d = q;
+            fe_norm<F>(d.y);
+            xyzz_mdbl<F>(acc, d); // acc == q
+        }
         else
             acc_id = true; // acc == -q
         return;
@@ -160,9 +167,9 @@ template <class F> HS_HD void xyzz_add(xyzz &r, const xyzz &p, const xyzz &q)
     fe_mul<F>(S2, q.Y, p.ZZZ);
     fe_sub<F>(P, U2, U1);
     fe_sub<F>(R, S2, S1);
-    if (fe_is_zero<F>(P))
+    if (fe_is_zero_norm<F>(P))
     {
-        if (fe_is_zero<F>(R))
+        if (fe_is_zero_norm<F>(R))
             xyzz_dbl<F>(r, p);
         else
             xyzz_identity<F>(r);

@@ -591,6 +591,21 @@ template <class F> HS_HD bool fe_is_zero(const fe &a)
     return fe_is_zero_canon<F>(a);
 }
 
+// The same test for the output of fe_norm / fe_add / fe_sub / reduce_cols, without the carry
+// pass: there only limbs 0..2 can equal 2^29, limbs 3..8 are strict digits, and the value is
+// below 2^261 + 2^88, so a multiple of m is k m with k <= 64.  A carry from limbs 0..2 changes
+// digits 5..7 only by running through digit 4 as 2^29 - 1 and leaving it 0, but every such k m
+// (k >= 1) has digit 4 nonzero (bits 133..144 of k m are all ones for both moduli); so for a
+// multiple of m the limbs 5..7 are its strict digits, and mixed limbs 5..7 prove the value
+// nonzero.
+template <class F> HS_HD bool fe_is_zero_norm(const fe &a)
+{
+    const uint32_t o = a.v[5] | a.v[6] | a.v[7], n = a.v[5] & a.v[6] & a.v[7];
+    if (o != 0 && n != M29)
+        return false;
+    return fe_is_zero_canon<F>(a);
+}
+
 template <class F> HS_HD bool fe_eq(const fe &a, const fe &b)
 {
     fe d;

@@ -460,20 +460,15 @@ __device__ __forceinline__ void add_entry_raw(xyzz &acc, bool &acc_id, const uin
     if (e == ENTRY_SKIP)
         return;
     const uint32_t xw[8] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w};
-    uint32_t yw[8] = {q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
-    // a negative digit adds -P: y -> m - y on the canonical words (branch-free, no normalisation)
-    const uint32_t keep = (e >> 31) ? 0u : 0xffffffffu;
-    int64_t br = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-    {
-        const int64_t x = (int64_t)F::mw(i) - yw[i] + br;
-        br = x >> 32;
-        yw[i] = (yw[i] & keep) | ((uint32_t)x & ~keep);
-    }
+    const uint32_t yw[8] = {q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
     aff p;
     fe_from_words(p.x, xw);
     fe_from_words(p.y, yw);
+    // a negative digit adds -P: y -> 65 m - y limb by limb (branch-free, lazy; xyzz_madd accepts it)
+    const uint32_t keep = (e >> 31) ? 0u : 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        p.y.v[i] = (p.y.v[i] & keep) | ((F::bias2(i) - p.y.v[i]) & ~keep);
     xyzz_madd<F>(acc, acc_id, p);
 }
 

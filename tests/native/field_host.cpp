@@ -105,7 +105,7 @@ extern "C" void hst_fe_op_raw(int field, int o, uint8_t *out40, const uint8_t *p
 }
 
 // ops on raw limbs (any normalised limbs, up to 2^29 each): result value (40 bytes, byte 39 = limb
-// bound flag) and, for o = 4, the zero test
+// bound flag); o = 4: the zero test, o = 5: the zero test for fe_norm / reduce_cols outputs
 extern "C" int hst_fe_op_limbs(int field, int o, uint8_t *out40, const uint32_t *la, const uint32_t *lb)
 {
     fe a, b, r;
@@ -116,6 +116,8 @@ extern "C" int hst_fe_op_limbs(int field, int o, uint8_t *out40, const uint32_t 
     }
     if (o == 4)
         return field == 0 ? (int)fe_is_zero<Fp>(a) : (int)fe_is_zero<Fq>(a);
+    if (o == 5)
+        return field == 0 ? (int)fe_is_zero_norm<Fp>(a) : (int)fe_is_zero_norm<Fq>(a);
     if (field == 0)
     {
         switch (o) { case 0: fe_mul<Fp>(r, a, b); break; case 1: fe_sq<Fp>(r, a); break; case 2: fe_add<Fp>(r, a, b); break; default: fe_sub<Fp>(r, a, b); }
@@ -247,7 +249,7 @@ template <class F> static void madd_chain(uint8_t *out, const uint8_t *pts, cons
         ld(q.x, pts + 64 * i);
         ld(q.y, pts + 64 * i + 32);
         if (neg[i])
-            fe_neg<F>(q.y, q.y);
+            fe_neg_lazy2<F>(q.y, q.y); // as the bucket kernels negate
         xyzz_madd<F>(acc, aid, q);
     }
     if (aid)

@@ -259,6 +259,41 @@ def test_field_extreme_limbs(L, k):
 
 
 @pytest.mark.parametrize("k", [0, 1])
+def test_zero_test_of_normalised_values(L, k):
+    """fe_is_zero_norm (no carry pass) on the values normalisation can produce: limbs 3..8 strict
+    29-bit digits, limbs 0..2 up to 2^29.  Every multiple j m (j <= 64) in every such form is
+    zero, and random / near-multiple values agree with the exact test."""
+    m = O.FIELD[k]
+    rnd = random.Random(91 + k)
+    top = 1 << 29
+    out = ctypes.create_string_buffer(40)
+    U = lambda l: (ctypes.c_uint32 * 9)(*l)
+
+    def forms(v):
+        d = [(v >> (29 * i)) & (top - 1) for i in range(9)]
+        if v >> 261:
+            return
+        yield d
+        # borrow one from digit p + 1 into digit p (p <= 2), while the digit stays <= 2^29
+        for p in range(3):
+            if d[p + 1] > 0 and d[p] == 0:
+                e = list(d)
+                e[p + 1] -= 1
+                e[p] += top
+                yield e
+
+    for j in range(65):
+        for d in forms(j * m):
+            assert L.hst_fe_op_limbs(k, 5, out, U(d), U([0] * 9)) == 1, (j, d)
+    for v in [1, m - 1, m + 1, 2 * m - 1, 1 << 255, 1 << 145, (1 << 261) - 1] + \
+             [j * m + rnd.choice([-1, 1]) * rnd.getrandbits(rnd.choice([3, 40, 140])) for j in range(1, 64)] + \
+             [rnd.getrandbits(261) for _ in range(2000)]:
+        for d in forms(v % (1 << 261)):
+            want = int(_limbs_value(d) % m == 0)
+            assert L.hst_fe_op_limbs(k, 5, out, U(d), U([0] * 9)) == want, hex(v)
+
+
+@pytest.mark.parametrize("k", [0, 1])
 def test_mul_sum2_extreme_limbs(L, k):
     """a b + c d with one reduction, as the point formulas form it: b = x - y with the small-limb
     bias, c = -z, every input at the representation's limits (limbs 0, 2^29 - 1 or 2^29); also
