@@ -206,8 +206,7 @@ __global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S,
     xyzz_to_jac_q<F>(acc, s);
     for (int w = FIXED_WINDOWS - 2; w >= 0; w--)
     {
-        for (int j = 0; j < 5; j++)
-            jac_dbl_q<F>(acc);
+        jac_dbl_n_q<F>(acc, 5);
         load_xyzz(s, sk + w);
         xyzz_to_jac_q<F>(t, s);
         jac_add_q<F>(acc, acc, t);

@@ -817,8 +817,7 @@ __global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out,
         acc = *total;
     for (; w >= w_lo; w--)
     {
-        for (int k = 0; k < c; k++)
-            jac_dbl_q<F>(acc);
+        jac_dbl_n_q<F>(acc, c);
         load_xyzz(s, win_out + w);
         xyzz_to_jac_q<F>(t, s);
         jac_add_q<F>(acc, acc, t);
@@ -1195,8 +1194,7 @@ __global__ void __launch_bounds__(64) k_horner_batch(const xyzz *__restrict__ wi
     xyzz_to_jac_q<F>(acc, s);
     for (int w = W - 2; w >= 0; w--)
     {
-        for (int k = 0; k < c; k++)
-            jac_dbl_q<F>(acc);
+        jac_dbl_n_q<F>(acc, c);
         load_xyzz(s, ws + w);
         xyzz_to_jac_q<F>(t, s);
         jac_add_q<F>(acc, acc, t);

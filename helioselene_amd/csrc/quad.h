@@ -98,6 +98,50 @@ template <class F> __device__ __forceinline__ void jac_dbl_q(jac &p)
     p.Z = Z3;
 }
 
+// n doublings of p (dbl-2001-b, same results as n calls of jac_dbl_q), pipelined across the
+// doublings: a doubling of (X, Y, Z) with delta = Z^2 and alpha = 3 (X - delta)(X + delta)
+// already known takes three product levels,
+//   A: gamma = Y^2, (Y + Z)^2, alpha^2                 -> Z3 = (Y + Z)^2 - gamma - delta
+//   B: beta = X gamma, gamma^2, delta3 = Z3^2          -> X3 = alpha^2 - 8 beta
+//   C: alpha (4 beta - X3), (X3 - delta3)(X3 + delta3) -> Y3, alpha3
+// since delta and alpha of the next doubling come out of levels B and C.  Two levels prepare the
+// first delta and alpha: 2 + 3n levels instead of 4n.
+template <class F> __device__ __forceinline__ void jac_dbl_n_q(jac &p, int n)
+{
+    if (n <= 0)
+        return;
+    fe d, a, xm, xp, t;
+    fe_sq<F>(d, p.Z);
+    fe_sub<F>(xm, p.X, d);
+    fe_add_lazy<F>(xp, p.X, d);
+    fe_mul<F>(t, xm, xp);
+    fe_mul_small<F>(a, t, 3);
+    for (int k = 0; k < n; k++)
+    {
+        // Operand bounds as in jac_dbl_q: Y + Z and X3 + delta3 are un-normalised sums (limbs
+        // <= 2^30), 4 beta - X3 a lazy subtraction multiplied by the normalised alpha.
+        fe yz, g, yz2, a2, Z3, b, g2, d3, X3, Y3, t3;
+        fe_add_lazy<F>(yz, p.Y, p.Z);
+        fe_mul_q3<F>(g, yz2, a2, p.Y, p.Y, yz, yz, a, a);
+        fe_sub2<F>(Z3, yz2, g, d);
+        fe_mul_q3<F>(b, g2, d3, p.X, g, g, g, Z3, Z3);
+        fe_mul_small<F>(t, b, 8);
+        fe_sub<F>(X3, a2, t);
+        fe_mul_small<F>(t, b, 4);
+        fe_sub_lazy<F>(t, t, X3);
+        fe_sub<F>(xm, X3, d3);
+        fe_add_lazy<F>(xp, X3, d3);
+        fe_mul_q2<F>(Y3, t3, a, t, xm, xp);
+        fe_mul_small<F>(t, g2, 8);
+        fe_sub<F>(Y3, Y3, t);
+        fe_mul_small<F>(a, t3, 3);
+        d = d3;
+        p.X = X3;
+        p.Y = Y3;
+        p.Z = Z3;
+    }
+}
+
 // xyzz -> Jacobian (Z' = ZZ ZZZ, X' = X ZZ ZZZ^2, Y' = (Y ZZ) Z'^2), three levels
 template <class F> __device__ __forceinline__ void xyzz_to_jac_q(jac &r, const xyzz &p)
 {
