@@ -67,9 +67,10 @@ template <class F> HS_HD void xyzz_dbl(xyzz &r, const xyzz &p)
     fe_mul<F>(W, U, V);
     fe_mul<F>(S, p.X, V);
     tangent_m<F>(M, p.X, p.ZZ);
-    fe X3, Y3;
-    fe_sq<F>(X3, M);
-    fe_sub2<F>(X3, X3, S, S); // M^2 - 2S
+    fe X3, Y3, z;
+    fe_zero(z);
+    fe_neg3_lazy<F>(t, S, S, z);
+    fe_sq_add<F>(X3, M, t); // M^2 - 2S
     fe_sub_lazy2<F>(t, S, X3);
     fe_neg_lazy2<F>(nY, p.Y);
     fe_mul_sum2<F>(Y3, M, t, nY, W); // M (S - X3) - W Y
@@ -89,9 +90,10 @@ template <class F> HS_HD void xyzz_mdbl(xyzz &r, const aff &p)
     fe_mul<F>(W, U, V);
     fe_mul<F>(S, p.x, V);
     tangent_m<F>(M, p.x, one);
-    fe X3, Y3;
-    fe_sq<F>(X3, M);
-    fe_sub2<F>(X3, X3, S, S); // M^2 - 2S
+    fe X3, Y3, z;
+    fe_zero(z);
+    fe_neg3_lazy<F>(t, S, S, z);
+    fe_sq_add<F>(X3, M, t); // M^2 - 2S
     fe_sub_lazy2<F>(t, S, X3);
     fe_neg_lazy2<F>(nY, p.y);
     fe_mul_sum2<F>(Y3, M, t, nY, W); // M (S - X3) - W y
@@ -114,11 +116,14 @@ template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
         acc_id = false;
         return;
     }
-    fe U2, S2, P, R;
-    fe_mul<F>(U2, q.x, acc.ZZ);
-    fe_mul<F>(S2, q.y, acc.ZZZ);
-    fe_sub<F>(P, U2, acc.X);
-    fe_sub<F>(R, S2, acc.Y);
+    // P = x2 ZZ1 - X1 and R = y2 ZZZ1 - Y1, each one product with the negation as addend
+    fe nX, nY, P, R;
+    fe_neg_lazy2<F>(nX, acc.X);
+    fe_neg_lazy2<F>(nY, acc.Y);
+    fe_mul_add<F>(P, q.x, acc.ZZ, nX);
+    fe_mul_add<F>(R, q.y, acc.ZZZ, nY);
+    fe_opaque(P);
+    fe_opaque(R);
     if (fe_is_zero_norm<F>(P))
     {
         if (fe_is_zero_norm<F>(R))
@@ -131,15 +136,14 @@ template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
             acc_id = true; // acc == -q
         return;
     }
-    fe PP, PPP, Q, t, nY;
+    fe PP, PPP, Q, t;
     fe_sq<F>(PP, P);
     fe_mul<F>(PPP, P, PP);
     fe_mul<F>(Q, acc.X, PP);
     fe X3, Y3;
-    fe_sq<F>(X3, R);
-    fe_sub3<F>(X3, X3, PPP, Q, Q); // R^2 - PPP - 2Q
+    fe_neg3_lazy<F>(t, PPP, Q, Q);
+    fe_sq_add<F>(X3, R, t); // R^2 - PPP - 2Q
     fe_sub_lazy2<F>(t, Q, X3);
-    fe_neg_lazy2<F>(nY, acc.Y);
     fe_mul_sum2<F>(Y3, R, t, nY, PPP); // R (Q - X3) - Y1 PPP
     fe_mul<F>(acc.ZZ, acc.ZZ, PP);
     fe_mul<F>(acc.ZZZ, acc.ZZZ, PPP);
@@ -160,13 +164,16 @@ template <class F> HS_HD void xyzz_add(xyzz &r, const xyzz &p, const xyzz &q)
         r = p;
         return;
     }
-    fe U1, U2, S1, S2, P, R;
+    // P = U2 - U1 and R = S2 - S1, the second product of each taking the negation as addend
+    fe U1, S1, nU1, nS1, P, R;
     fe_mul<F>(U1, p.X, q.ZZ);
-    fe_mul<F>(U2, q.X, p.ZZ);
     fe_mul<F>(S1, p.Y, q.ZZZ);
-    fe_mul<F>(S2, q.Y, p.ZZZ);
-    fe_sub<F>(P, U2, U1);
-    fe_sub<F>(R, S2, S1);
+    fe_neg_lazy2<F>(nU1, U1);
+    fe_neg_lazy2<F>(nS1, S1);
+    fe_mul_add<F>(P, q.X, p.ZZ, nU1);
+    fe_mul_add<F>(R, q.Y, p.ZZZ, nS1);
+    fe_opaque(P);
+    fe_opaque(R);
     if (fe_is_zero_norm<F>(P))
     {
         if (fe_is_zero_norm<F>(R))
@@ -175,16 +182,15 @@ template <class F> HS_HD void xyzz_add(xyzz &r, const xyzz &p, const xyzz &q)
             xyzz_identity<F>(r);
         return;
     }
-    fe PP, PPP, Q, t, nY;
+    fe PP, PPP, Q, t;
     fe_sq<F>(PP, P);
     fe_mul<F>(PPP, P, PP);
     fe_mul<F>(Q, U1, PP);
     fe X3, Y3;
-    fe_sq<F>(X3, R);
-    fe_sub3<F>(X3, X3, PPP, Q, Q); // R^2 - PPP - 2Q
+    fe_neg3_lazy<F>(t, PPP, Q, Q);
+    fe_sq_add<F>(X3, R, t); // R^2 - PPP - 2Q
     fe_sub_lazy2<F>(t, Q, X3);
-    fe_neg_lazy2<F>(nY, S1);
-    fe_mul_sum2<F>(Y3, R, t, nY, PPP); // R (Q - X3) - S1 PPP
+    fe_mul_sum2<F>(Y3, R, t, nS1, PPP); // R (Q - X3) - S1 PPP
     fe ZZ3, ZZZ3;
     fe_mul<F>(ZZ3, p.ZZ, q.ZZ);
     fe_mul<F>(ZZ3, ZZ3, PP);

@@ -34,6 +34,7 @@ struct fe
 
 HS_HD uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
 
+
 struct Fp
 {
     static constexpr int NR = 1; // limbs of R = 2^261 mod p
@@ -163,6 +164,19 @@ struct Fq
     }
     static constexpr int NCW = 4;
 };
+
+// Hides what the compiler knows about the bits of a value's limbs.  Given visible limb bounds it
+// can split a following product into 32-bit carry chains with many extra register moves.
+HS_HD void fe_opaque(fe &a)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        asm volatile("" : "+v"(a.v[i]));
+#else
+    (void)a;
+#endif
+}
 
 HS_HD void fe_zero(fe &r)
 {
@@ -300,6 +314,15 @@ template <class F> HS_HD void fe_neg_lazy2(fe &r, const fe &a)
 #pragma unroll
     for (int i = 0; i < NL; i++)
         r.v[i] = F::bias2(i) - a.v[i];
+}
+
+// r = 257 m - b - c - d for normalised b, c, d: limbs in (2^29, 2^31 + 2^29), an addend for
+// fe_mul_add / fe_sq_add
+template <class F> HS_HD void fe_neg3_lazy(fe &r, const fe &b, const fe &c, const fe &d)
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = F::bias3(i) - b.v[i] - c.v[i] - d.v[i];
 }
 
 template <class F> HS_HD void fe_neg(fe &r, const fe &a)
@@ -441,6 +464,44 @@ template <class F> HS_HD void fe_mul_sum2(fe &r, const fe &a, const fe &b, const
             w[i + j] = mad64(a.v[i], b.v[j], w[i + j]);
             w[i + j] = mad64(c.v[i], d.v[j], w[i + j]);
         }
+    reduce_cols<F>(r, w);
+}
+
+// r = a b + c and r = a^2 + c with one reduction: c (limbs < 2^32, such as a lazy negation)
+// starts the nine low columns, so a product followed by a subtraction needs no second
+// normalisation.  Operands of the product as for fe_mul.
+template <class F> HS_HD void fe_mul_add(fe &r, const fe &a, const fe &b, const fe &c)
+{
+    uint64_t w[2 * NL - 1];
+#pragma unroll
+    for (int k = 0; k < 2 * NL - 1; k++)
+        w[k] = k < NL ? c.v[k] : 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+#pragma unroll
+        for (int j = 0; j < NL; j++)
+            w[i + j] = mad64(a.v[i], b.v[j], w[i + j]);
+    reduce_cols<F>(r, w);
+}
+
+template <class F> HS_HD void fe_sq_add(fe &r, const fe &a, const fe &c)
+{
+    uint64_t w[2 * NL - 1];
+    uint32_t d[NL];
+#pragma unroll
+    for (int k = 0; k < 2 * NL - 1; k++)
+        w[k] = k < NL ? c.v[k] : 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        d[i] = a.v[i] << 1;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        w[2 * i] = mad64(a.v[i], a.v[i], w[2 * i]);
+#pragma unroll
+        for (int j = i + 1; j < NL; j++)
+            w[i + j] = mad64(a.v[i], d[j], w[i + j]);
+    }
     reduce_cols<F>(r, w);
 }
 
