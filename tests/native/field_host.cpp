@@ -171,6 +171,46 @@ extern "C" void hst_fe_mul_sum2(int field, uint8_t *out40, const uint32_t *la, c
         mul_sum2_limbs<Fq>(out40, la, lb, lb2, lc, ld, lazy);
 }
 
+// mode 0: a b + fe_neg_lazy2(c); mode 1: a^2 + fe_neg3_lazy(c, d, e) (out40 as hst_fe_op_limbs)
+template <class F>
+static void mul_add_limbs(uint8_t *out40, const uint32_t *la, const uint32_t *lb, const uint32_t *lc,
+                          const uint32_t *ld, const uint32_t *le, int mode)
+{
+    fe a, b, c, d, e, t, r;
+    for (int i = 0; i < NL; i++)
+    {
+        a.v[i] = la[i];
+        b.v[i] = lb[i];
+        c.v[i] = lc[i];
+        d.v[i] = ld[i];
+        e.v[i] = le[i];
+    }
+    if (mode == 0)
+    {
+        fe_neg_lazy2<F>(t, c);
+        fe_mul_add<F>(r, a, b, t);
+    }
+    else
+    {
+        fe_neg3_lazy<F>(t, c, d, e);
+        fe_sq_add<F>(r, a, t);
+    }
+    st_raw(out40, r);
+    uint32_t mx = 0;
+    for (int i = 0; i < NL; i++)
+        mx = r.v[i] > mx ? r.v[i] : mx;
+    out40[39] = mx > (1u << 29) ? 1 : 0;
+}
+
+extern "C" void hst_fe_mul_add(int field, uint8_t *out40, const uint32_t *la, const uint32_t *lb, const uint32_t *lc,
+                               const uint32_t *ld, const uint32_t *le, int mode)
+{
+    if (field == 0)
+        mul_add_limbs<Fp>(out40, la, lb, lc, ld, le, mode);
+    else
+        mul_add_limbs<Fq>(out40, la, lb, lc, ld, le, mode);
+}
+
 extern "C" void hst_fe_op(int field, int o, uint8_t *out, const uint8_t *a, const uint8_t *b)
 {
     g_field = field;

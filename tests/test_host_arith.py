@@ -322,6 +322,34 @@ def test_mul_sum2_extreme_limbs(L, k):
             assert v < (1 << 262) and v % m == want % m
 
 
+@pytest.mark.parametrize("k", [0, 1])
+def test_mul_add_extreme_limbs(L, k):
+    """a b - c and a^2 - c - d - e with one reduction (the lazy negation as the low columns' start
+    value), every input at the representation's limits, and one product operand a lazily negated
+    base coordinate (limbs up to 2^30 - 2)."""
+    m = O.FIELD[k]
+    rnd = random.Random(55 + k)
+    top = 1 << 29
+    pats = [[top] * 9, [top - 1] * 9, [0] * 9, [top, 0] * 4 + [top]]
+    pats += [[rnd.choice([0, top, top - 1, rnd.getrandbits(29)]) for _ in range(9)] for _ in range(300)]
+    out = ctypes.create_string_buffer(40)
+    U = lambda l: (ctypes.c_uint32 * 9)(*l)
+    for i, la in enumerate(pats):
+        b, c, d, e = (pats[(i * s + 1) % len(pats)] for s in (3, 5, 7, 11))
+        A, B, C, D, E = (_limbs_value(v) for v in (la, b, c, d, e))
+        L.hst_fe_mul_add(k, out, U(la), U(b), U(c), U(d), U(e), 0)
+        v = int.from_bytes(out.raw[:39], "little")
+        assert out.raw[39] == 0 and v < (1 << 262) and v % m == (A * B - C) % m, i
+        L.hst_fe_mul_add(k, out, U(la), U(b), U(c), U(d), U(e), 1)
+        v = int.from_bytes(out.raw[:39], "little")
+        assert out.raw[39] == 0 and v < (1 << 262) and v % m == (A * A - C - D - E) % m, i
+    lazy = [(1 << 30) - 2] * 9  # 65 m - y for a base with zero limbs: the largest lazy operand
+    for b in pats[:4]:
+        L.hst_fe_mul_add(k, out, U(lazy), U(b), U(pats[0]), U(pats[0]), U(pats[0]), 0)
+        v = int.from_bytes(out.raw[:39], "little")
+        assert out.raw[39] == 0 and v % m == (_limbs_value(lazy) * _limbs_value(b) - _limbs_value(pats[0])) % m
+
+
 @pytest.mark.parametrize("c", [0, 1])
 def test_jacobian_chain_vs_oracle(L, c):
     """xyzz -> Jacobian conversion, dbl-2001-b chains and add-2007-bl (including P + P and
