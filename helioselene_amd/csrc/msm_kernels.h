@@ -2,10 +2,12 @@
 //
 // Pipeline for  Q = sum s_i P_i  (reference helios/src/x64/helios_msm_vartime.cpp:265-388):
 //   ingest   radix-2^51 Jacobian points (the reference struct) -> canonical affine, identity flags
-//   recode   scalars -> signed width-c digits for W = ceil(256/c) windows, bucket histogram
-//   sort     per-window counting sort of (window, bucket) keys: scan of the histogram + scatter
-//   bucket   per-bucket sums in xyzz with mixed additions (thread per bucket; one workgroup per
-//            oversized bucket with an LDS tree)
+//   recode   scalars -> signed width-c digits for W = ceil(256/c) windows, with a coarse
+//            (window, super-bucket) histogram per block
+//   sort     two-level LDS counting sort of the (window, bucket) keys: scan of the coarse
+//            histogram, coarse scatter, then one block per region orders it by bucket
+//   bucket   xyzz mixed additions over fixed-size runs of the sorted entries (every thread the
+//            same work), then merges of the buckets cut by run edges
 //   reduce   per-window running sums over segments of buckets, then a tree over segments
 //   combine  Horner over windows (c doublings per window), affine + 32-byte encoding
 #pragma once
@@ -160,37 +162,6 @@ __global__ void __launch_bounds__(256) k_ingest(const uint64_t *__restrict__ jac
     store_aff<F>(out, i, a);
 }
 
-// ---------------------------------------------------------------- recode + histogram
-// digits[w * n + i]: 0, or |d| | sign << 31;  counts[w * B + |d| - 1] += 1
-template <class C>
-__global__ void __launch_bounds__(256) k_recode(const uint32_t *__restrict__ scalars, const uint8_t *__restrict__ is_id,
-                                                uint32_t n, int c, int W, int w8_compat,
-                                                uint32_t *__restrict__ digits, uint32_t *__restrict__ counts)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    uint32_t s[8];
-    const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * i);
-    uint4 a = sp[0], b = sp[1];
-    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
-    s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
-    uint32_t k[8];
-    scalar_prepare<typename C::O>(k, s, w8_compat != 0);
-    const bool skip = is_id[i] != 0;
-    const uint32_t B = 1u << (c - 1);
-    uint32_t carry = 0;
-    for (int w = 0; w < W; w++)
-    {
-        uint32_t d = recode_digit(k, c, w, carry);
-        if (skip)
-            d = 0;
-        digits[(size_t)w * n + i] = d;
-        if (d)
-            atomicAdd(&counts[(size_t)w * B + (d & 0x7fffffffu) - 1], 1u);
-    }
-}
-
 // ---------------------------------------------------------------- exclusive scan (uint32)
 // Three-phase scan: per-block scan of 2048 elements -> block sums -> add offsets.
 constexpr int SCAN_BLOCK = 1024, SCAN_ITEMS = 2;
@@ -268,24 +239,6 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(uint32_t *__restrict__ 
     for (int k = 0; k < SCAN_ITEMS; k++)
         if (base + k < len)
             out[base + k] += off;
-}
-
-// ---------------------------------------------------------------- scatter into sorted order
-__global__ void __launch_bounds__(256) k_scatter(const uint32_t *__restrict__ digits, uint32_t n, int W, uint32_t B,
-                                                 uint32_t *__restrict__ cursor, uint32_t *__restrict__ sorted)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    for (int w = 0; w < W; w++)
-    {
-        uint32_t d = digits[(size_t)w * n + i];
-        if (d)
-        {
-            uint32_t pos = atomicAdd(&cursor[(size_t)w * B + (d & 0x7fffffffu) - 1], 1u);
-            sorted[pos] = i | (d & 0x80000000u);
-        }
-    }
 }
 
 // ---------------------------------------------------------------- two-level counting sort
@@ -428,19 +381,6 @@ __global__ void __launch_bounds__(SORT_FINE_THREADS) k_fine_sort(const uint32_t 
 // ---------------------------------------------------------------- bucket sums
 constexpr uint32_t ENTRY_SKIP = 0xffffffffu; // padding entry of the batched layout
 
-template <class C>
-__device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const uint32_t *__restrict__ pts, uint32_t e)
-{
-    using F = typename C::F;
-    if (e == ENTRY_SKIP)
-        return;
-    aff p;
-    load_aff(p, pts, e & 0x7fffffffu);
-    if (e >> 31)
-        fe_neg<F>(p.y, p.y);
-    xyzz_madd<F>(acc, acc_id, p);
-}
-
 // Raw 64-byte base of an entry (point 0 for padding, which add_entry_raw then skips); issued one
 // entry ahead in the flat kernel so the gather overlaps the current addition.
 __device__ __forceinline__ void load_entry_raw(uint4 (&q)[4], const uint32_t *__restrict__ pts, uint32_t e)
@@ -472,34 +412,6 @@ __device__ __forceinline__ void add_entry_raw(xyzz &acc, bool &acc_id, const uin
     xyzz_madd<F>(acc, acc_id, p);
 }
 
-// one thread per bucket with at most `lmax` entries; larger buckets are queued for k_bucket_big
-template <class C>
-__global__ void __launch_bounds__(256) k_bucket_small(const uint32_t *__restrict__ offsets, uint32_t nbuckets,
-                                                      const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ pts,
-                                                      uint32_t lmax, xyzz *__restrict__ buckets,
-                                                      uint32_t *__restrict__ big_list, uint32_t *__restrict__ big_count)
-{
-    using F = typename C::F;
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nbuckets)
-        return;
-    const uint32_t beg = offsets[g], end = offsets[g + 1], cnt = end - beg;
-    if (cnt > lmax)
-    {
-        uint32_t k = atomicAdd(big_count, 1u);
-        big_list[k] = g;
-        return;
-    }
-    xyzz acc;
-    bool acc_id = true;
-    xyzz_identity<F>(acc);
-    for (uint32_t j = beg; j < end; j++)
-        add_entry<C>(acc, acc_id, pts, sorted[j]);
-    if (acc_id)
-        xyzz_identity<F>(acc);
-    store_xyzz(buckets + g, acc);
-}
-
 constexpr int BIG_THREADS = 256;
 
 // LDS tree sum of one xyzz per thread; result in thread 0.  sh holds BIG_THREADS points.
@@ -523,36 +435,6 @@ template <class C> __device__ void block_tree_sum(xyzz &acc, xyzz *sh)
     }
     if (t == 0)
         load_xyzz(acc, sh);
-}
-
-// one workgroup per oversized bucket: strided per-thread sums, then an LDS tree
-template <class C>
-__global__ void __launch_bounds__(BIG_THREADS) k_bucket_big(const uint32_t *__restrict__ offsets,
-                                                            const uint32_t *__restrict__ sorted,
-                                                            const uint32_t *__restrict__ pts,
-                                                            const uint32_t *__restrict__ big_list,
-                                                            const uint32_t *__restrict__ big_count,
-                                                            xyzz *__restrict__ buckets)
-{
-    using F = typename C::F;
-    __shared__ xyzz sh[BIG_THREADS];
-    const uint32_t nbig = *big_count;
-    for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x)
-    {
-        const uint32_t g = big_list[k];
-        const uint32_t beg = offsets[g], end = offsets[g + 1];
-        xyzz acc;
-        bool acc_id = true;
-        xyzz_identity<F>(acc);
-        for (uint32_t j = beg + threadIdx.x; j < end; j += blockDim.x)
-            add_entry<C>(acc, acc_id, pts, sorted[j]);
-        if (acc_id)
-            xyzz_identity<F>(acc);
-        block_tree_sum<C>(acc, sh);
-        if (threadIdx.x == 0)
-            store_xyzz(buckets + g, acc);
-        __syncthreads();
-    }
 }
 
 // ---------------------------------------------------------------- flat, balanced bucket sums
