@@ -384,7 +384,8 @@ template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
         t >>= 29;
         h[NL - 1] = (uint32_t)t & M29;
         h[NL] = (uint32_t)(t >> 29);
-        // first fold into columns 0..13 (each < 2^61.2 + 5 * 2^58 < 2^62)
+        // first fold into columns 0..13: each gains at most five products h_i R_j (< 2^58 each)
+        // on top of its < 45 * 2^58
         uint64_t w[NL + F::NR];
 #pragma unroll
         for (int k = 0; k < NL + F::NR; k++)
@@ -394,8 +395,9 @@ template <class F> HS_HD void reduce_cols(fe &r, const uint64_t c[2 * NL - 1])
 #pragma unroll
             for (int j = 0; j < F::NR; j++)
                 w[i + j] = mad64(h[i], F::r(j), w[i + j]);
-        // columns 9..13 again as one number M (< 2^134 since L + R H < 2^395), carried into
-        // digits m[0..4]; the second fold m_i R_j (i + j <= 8) lands in columns 0..8
+        // columns 9..13 again as one number M (< 2^139 since H < 2^267 and L + R H < 2^400),
+        // carried into digits m[0..4]; the second fold m_i R_j (i + j <= 8) lands in columns 0..8,
+        // five more products at most
         uint32_t m[F::NR];
         t = 0;
 #pragma unroll
