@@ -541,6 +541,20 @@ template <class F> HS_HD void fe_mul_small(fe &r, const fe &a, uint32_t k)
     fold_top<F>(r, (uint32_t)(t >> 29));
 }
 
+// r = a - k b for normalised a, b and small k (<= 16) with one carry pass: limb i is
+// a_i + k (bias2_i - b_i) < 2^29 + k 2^30, so the carries stay below 2^6
+template <class F> HS_HD void fe_sub_small_mul(fe &r, const fe &a, const fe &b, uint32_t k)
+{
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        t = mad64(F::bias2(i) - b.v[i], k, (uint64_t)a.v[i] + (t >> 29));
+        r.v[i] = (uint32_t)t & M29;
+    }
+    fold_top<F>(r, (uint32_t)(t >> 29));
+}
+
 // ---------------------------------------------------------------- conversions
 // 8 little-endian words (any value < 2^256) -> limbs (normalised, not reduced)
 HS_HD void fe_from_words(fe &r, const uint32_t w[8])

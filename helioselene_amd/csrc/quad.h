@@ -125,15 +125,13 @@ template <class F> __device__ __forceinline__ void jac_dbl_n_q(jac &p, int n)
         fe_mul_q3<F>(g, yz2, a2, p.Y, p.Y, yz, yz, a, a);
         fe_sub2<F>(Z3, yz2, g, d);
         fe_mul_q3<F>(b, g2, d3, p.X, g, g, g, Z3, Z3);
-        fe_mul_small<F>(t, b, 8);
-        fe_sub<F>(X3, a2, t);
+        fe_sub_small_mul<F>(X3, a2, b, 8);
         fe_mul_small<F>(t, b, 4);
         fe_sub_lazy<F>(t, t, X3);
         fe_sub<F>(xm, X3, d3);
         fe_add_lazy<F>(xp, X3, d3);
         fe_mul_q2<F>(Y3, t3, a, t, xm, xp);
-        fe_mul_small<F>(t, g2, 8);
-        fe_sub<F>(Y3, Y3, t);
+        fe_sub_small_mul<F>(Y3, Y3, g2, 8);
         fe_mul_small<F>(a, t3, 3);
         d = d3;
         p.X = X3;

@@ -118,6 +118,19 @@ extern "C" int hst_fe_op_limbs(int field, int o, uint8_t *out40, const uint32_t 
         return field == 0 ? (int)fe_is_zero<Fp>(a) : (int)fe_is_zero<Fq>(a);
     if (o == 5)
         return field == 0 ? (int)fe_is_zero_norm<Fp>(a) : (int)fe_is_zero_norm<Fq>(a);
+    if (o >= 16 && o < 32) // a - k b, k = o - 16
+    {
+        if (field == 0)
+            fe_sub_small_mul<Fp>(r, a, b, (uint32_t)(o - 16));
+        else
+            fe_sub_small_mul<Fq>(r, a, b, (uint32_t)(o - 16));
+        st_raw(out40, r);
+        uint32_t mx = 0;
+        for (int i = 0; i < NL; i++)
+            mx = r.v[i] > mx ? r.v[i] : mx;
+        out40[39] = mx > (1u << 29) ? 1 : 0;
+        return 0;
+    }
     if (field == 0)
     {
         switch (o) { case 0: fe_mul<Fp>(r, a, b); break; case 1: fe_sq<Fp>(r, a); break; case 2: fe_add<Fp>(r, a, b); break; default: fe_sub<Fp>(r, a, b); }

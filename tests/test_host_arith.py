@@ -323,6 +323,25 @@ def test_mul_sum2_extreme_limbs(L, k):
 
 
 @pytest.mark.parametrize("k", [0, 1])
+def test_sub_small_mul_extreme_limbs(L, k):
+    """a - j b with one carry pass (j = 1..15) for normalised inputs at the limb limits."""
+    m = O.FIELD[k]
+    rnd = random.Random(66 + k)
+    top = 1 << 29
+    pats = [[top] * 9, [top - 1] * 9, [0] * 9, [top, 0] * 4 + [top], [0, top] * 4 + [0]]
+    pats += [[rnd.choice([0, top, top - 1, rnd.getrandbits(29)]) for _ in range(9)] for _ in range(120)]
+    out = ctypes.create_string_buffer(40)
+    U = lambda l: (ctypes.c_uint32 * 9)(*l)
+    for i, la in enumerate(pats):
+        lb = pats[(i * 7 + 3) % len(pats)]
+        A, B = _limbs_value(la), _limbs_value(lb)
+        for j in (1, 2, 3, 4, 8, 15):
+            L.hst_fe_op_limbs(k, 16 + j, out, U(la), U(lb))
+            v = int.from_bytes(out.raw[:39], "little")
+            assert out.raw[39] == 0 and v < (1 << 262) and v % m == (A - j * B) % m, (i, j)
+
+
+@pytest.mark.parametrize("k", [0, 1])
 def test_mul_add_extreme_limbs(L, k):
     """a b - c and a^2 - c - d - e with one reduction (the lazy negation as the low columns' start
     value), every input at the representation's limits, and one product operand a lazily negated
