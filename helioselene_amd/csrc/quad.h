@@ -60,6 +60,18 @@ __device__ __forceinline__ void fe_mul_q3(fe &r0, fe &r1, fe &r2, const fe &a0, 
     fe_qbcast<2>(r2, r);
 }
 
+// three independent squares a_k^2, one per lane (lanes 2 and 3 both square a2)
+template <class F>
+__device__ __forceinline__ void fe_sq_q3(fe &r0, fe &r1, fe &r2, const fe &a0, const fe &a1, const fe &a2)
+{
+    fe a, r;
+    fe_qsel(a, a0, a1, a2, a2);
+    fe_sq<F>(r, a);
+    fe_qbcast<0>(r0, r);
+    fe_qbcast<1>(r1, r);
+    fe_qbcast<2>(r2, r);
+}
+
 template <class F>
 __device__ __forceinline__ void fe_mul_q2(fe &r0, fe &r1, const fe &a0, const fe &b0, const fe &a1, const fe &b1)
 {
@@ -79,7 +91,7 @@ template <class F> __device__ __forceinline__ void jac_dbl_q(jac &p)
     // for both operands; 4 beta - X3 is a lazy subtraction, multiplied by the normalised alpha.
     fe yz, delta, gamma, yz2, xm, xp, beta, t, g2, alpha, X3, Y3, Z3, t0;
     fe_add_lazy<F>(yz, p.Y, p.Z);
-    fe_mul_q3<F>(delta, gamma, yz2, p.Z, p.Z, p.Y, p.Y, yz, yz);
+    fe_sq_q3<F>(delta, gamma, yz2, p.Z, p.Y, yz);
     fe_sub<F>(xm, p.X, delta);
     fe_add_lazy<F>(xp, p.X, delta);
     fe_mul_q3<F>(beta, t, g2, p.X, gamma, xm, xp, gamma, gamma);
@@ -122,7 +134,7 @@ template <class F> __device__ __forceinline__ void jac_dbl_n_q(jac &p, int n)
         // <= 2^30), 4 beta - X3 a lazy subtraction multiplied by the normalised alpha.
         fe yz, g, yz2, a2, Z3, b, g2, d3, X3, Y3, t3;
         fe_add_lazy<F>(yz, p.Y, p.Z);
-        fe_mul_q3<F>(g, yz2, a2, p.Y, p.Y, yz, yz, a, a);
+        fe_sq_q3<F>(g, yz2, a2, p.Y, yz, a);
         fe_sub2<F>(Z3, yz2, g, d);
         fe_mul_q3<F>(b, g2, d3, p.X, g, g, g, Z3, Z3);
         fe_sub_small_mul<F>(X3, a2, b, 8);
@@ -153,28 +165,28 @@ template <class F> __device__ __forceinline__ void xyzz_to_jac_q(jac &r, const x
 // add-2007-bl with the identity / equal / opposite cases, same results as jac_add
 template <class F> __device__ __forceinline__ void jac_add_q(jac &r, const jac &p, const jac &q)
 {
-    if (fe_is_zero<F>(p.Z))
+    if (fe_is_zero_norm<F>(p.Z))
     {
         r = q;
         return;
     }
-    if (fe_is_zero<F>(q.Z))
+    if (fe_is_zero_norm<F>(q.Z))
     {
         r = p;
         return;
     }
     fe Z1Z1, Z2Z2, zs, zs2, U1, U2, a, b, S1, S2, H, rr, I, Zt, Z3, J, V, r2, X3, Y3, t0, t1;
     fe_add_lazy<F>(zs, p.Z, q.Z); // squared below: limbs <= 2^30 are fine for both operands
-    fe_mul_q3<F>(Z1Z1, Z2Z2, zs2, p.Z, p.Z, q.Z, q.Z, zs, zs);
+    fe_sq_q3<F>(Z1Z1, Z2Z2, zs2, p.Z, q.Z, zs);
     fe_mul_q4<F>(U1, U2, a, b, p.X, Z2Z2, q.X, Z1Z1, q.Z, Z2Z2, p.Z, Z1Z1);
     fe_sub<F>(H, U2, U1);
     fe_add_lazy<F>(t0, H, H);
     fe_sub2<F>(Zt, zs2, Z1Z1, Z2Z2);
     fe_mul_q4<F>(S1, S2, I, Z3, p.Y, a, q.Y, b, t0, t0, Zt, H);
     fe_sub<F>(rr, S2, S1);
-    if (fe_is_zero<F>(H))
+    if (fe_is_zero_norm<F>(H))
     {
-        if (fe_is_zero<F>(rr))
+        if (fe_is_zero_norm<F>(rr))
         {
             r = p;
             jac_dbl_q<F>(r);
