@@ -336,6 +336,12 @@ int scan_exclusive(Workspace &cx, const uint32_t *in, uint32_t *out, uint32_t le
     return 0;
 }
 
+// windows per k_reduce_win workgroup (256 threads, min(segments, 256) of them per window)
+static uint32_t reduce_win_per_block(uint32_t segs_per_win)
+{
+    return segs_per_win < (uint32_t)hs::BIG_THREADS ? (uint32_t)hs::BIG_THREADS / segs_per_win : 1u;
+}
+
 // The plain MSM on device-resident inputs.  d_scalars: n x 8 words; d_points: n x 15 uint64.
 // Point indices in the sort entries have 24 bits: larger MSMs run as chunks (msm_device_any).
 constexpr size_t MSM_MAX_CHUNK = (size_t)1 << 23;
@@ -431,8 +437,9 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         const uint32_t nseg = (uint32_t)(w_hi - w_lo) * segs_per_win;
         hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((nseg + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G,
                            (uint32_t)w_lo, nseg, seg);
-        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(w_hi - w_lo), dim3(hs::BIG_THREADS), 0, st, seg, segs_per_win,
-                           (uint32_t)w_lo, win);
+        const uint32_t nwin = (uint32_t)(w_hi - w_lo), wpb = reduce_win_per_block(segs_per_win);
+        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((nwin + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg,
+                           segs_per_win, (uint32_t)w_lo, nwin, win);
         cx.mark(st, 4);
         if (Q > 1)
         {
@@ -530,7 +537,9 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
                        g.bnd.as<hs::xyzz>(), g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>(), buckets);
     hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((g.nseg + 255) / 256), blk, 0, st, buckets, offsets, g.B, g.G, 0u, g.nseg,
                        seg);
-    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(g.V), dim3(hs::BIG_THREADS), 0, st, seg, g.B / g.G, 0u, win);
+    const uint32_t wpb = reduce_win_per_block(g.B / g.G);
+    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((g.V + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, g.B / g.G, 0u,
+                       g.V, win);
     hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((4 * g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
                        g.total.as<hs::jac>());
     hipLaunchKernelGGL(hs::k_finalize_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, g.total.as<hs::jac>(), d, g.M);

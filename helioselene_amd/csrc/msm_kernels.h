@@ -627,25 +627,50 @@ __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buc
     store_xyzz(seg_out + t, T);
 }
 
-// one workgroup per window: sum of the window's segment results
+// Sums of the windows' segment results.  A window of segs_per_win segments (a power of two) gets
+// tw = min(segs_per_win, 256) threads, so one workgroup covers 256 / tw windows: each thread adds
+// a strided share of its window's segments, then an LDS tree over the tw threads of the window.
+// (Batches of small MSMs have windows of only a few segments; one workgroup each left almost all
+// of its threads summing identities.)
 template <class C>
 __global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win,
-                                                            uint32_t w_lo, xyzz *__restrict__ win_out)
+                                                            uint32_t w_lo, uint32_t nwin, xyzz *__restrict__ win_out)
 {
     using F = typename C::F;
     __shared__ xyzz sh[BIG_THREADS];
-    const uint32_t w = w_lo + blockIdx.x;
+    const uint32_t tw = segs_per_win < (uint32_t)BIG_THREADS ? segs_per_win : (uint32_t)BIG_THREADS;
+    const uint32_t per_blk = BIG_THREADS / tw, lw = threadIdx.x / tw, lt = threadIdx.x % tw;
+    const uint32_t wl = blockIdx.x * per_blk + lw; // window index within the launch
+    const bool live = wl < nwin;
+    const uint32_t w = w_lo + wl;
     xyzz acc;
     xyzz_identity<F>(acc);
-    for (uint32_t s = threadIdx.x; s < segs_per_win; s += blockDim.x)
+    if (live)
+        for (uint32_t s = lt; s < segs_per_win; s += tw)
+        {
+            xyzz b;
+            load_xyzz(b, seg_out + (size_t)w * segs_per_win + s);
+            xyzz_add<F>(acc, acc, b);
+        }
+    store_xyzz(sh + threadIdx.x, acc);
+    __syncthreads();
+    for (uint32_t h = tw >> 1; h > 0; h >>= 1)
     {
-        xyzz b;
-        load_xyzz(b, seg_out + (size_t)w * segs_per_win + s);
-        xyzz_add<F>(acc, acc, b);
+        if (lt < h)
+        {
+            xyzz x, y;
+            load_xyzz(x, sh + threadIdx.x);
+            load_xyzz(y, sh + threadIdx.x + h);
+            xyzz_add<F>(x, x, y);
+            store_xyzz(sh + threadIdx.x, x);
+        }
+        __syncthreads();
     }
-    block_tree_sum<C>(acc, sh);
-    if (threadIdx.x == 0)
+    if (lt == 0 && live)
+    {
+        load_xyzz(acc, sh + threadIdx.x);
         store_xyzz(win_out + w, acc);
+    }
 }
 
 // ---------------------------------------------------------------- window combine + encode
