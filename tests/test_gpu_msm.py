@@ -139,6 +139,10 @@ def test_edge_cases(c):
     sc = (77).to_bytes(32, "little") * n
     out, _ = run(c, sc, pts)
     assert out == O.scalar_mul(c, (77 * n).to_bytes(32, "little"), g)
+    # the same with negative digits: the doubling path gets a lazily negated base
+    sc = (order - 77).to_bytes(32, "little") * n
+    out, _ = run(c, sc, pts)
+    assert out == O.scalar_mul(c, ((order - 77) * n % order).to_bytes(32, "little"), g)
     # non-canonical scalars: order + k and 2^256 - 1
     big = [(order + 5), (1 << 256) - 1, (1 << 255) + 3, order]
     sc = b"".join(v.to_bytes(32, "little") for v in big) * (n // 4)
