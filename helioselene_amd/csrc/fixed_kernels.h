@@ -14,6 +14,10 @@
 // recoding differs from helios_scalar_recode_signed5 digit by digit but represents the same
 // integer, and group arithmetic here is exact, so every result is the same point.
 //
+// Resident sets built from points use the same scheme with 8-bit windows (128 multiples per
+// base, 32 windows, 8 doublings per window); the drop-in msm_fixed keeps the reference's width-5
+// tables.
+//
 // Table entries are canonical affine (x, y) as 16 little-endian words; an all-zero entry is the
 // identity (the table of an identity base).
 #pragma once
@@ -25,17 +29,26 @@ namespace hs
 constexpr int FIXED_WINDOWS = 52;
 constexpr int FIXED_ENTRIES = 16;
 
+// Table geometry for window width WB: 2^(WB-1) multiples per base, ceil(256 / WB) signed windows.
+// WB = 5 is the reference's layout (the drop-in tables); resident sets built from points use
+// WB = 8 (32 additions per base and MSM instead of 52, 8 KiB of table per base).
+template <int WB> struct FixedW
+{
+    static constexpr int entries = 1 << (WB - 1);
+    static constexpr int windows = (256 + WB - 1) / WB;
+};
+
 // Tables from reference Jacobian structs: thread (i, k) computes (k+1) * P_i.  Optionally also
 // writes the reference's helios_affine layout (x[5], y[5] radix-2^51, canonical).
-template <class C>
+template <class C, int ENTRIES>
 __global__ void __launch_bounds__(256) k_fixed_tables(const uint64_t *__restrict__ pts, uint32_t n,
                                                       uint32_t *__restrict__ tables, uint64_t *__restrict__ ref_out)
 {
     using F = typename C::F;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * FIXED_ENTRIES)
+    if (t >= n * ENTRIES)
         return;
-    const uint32_t i = t / FIXED_ENTRIES, k = t % FIXED_ENTRIES;
+    const uint32_t i = t / ENTRIES, k = t % ENTRIES;
     const uint64_t *p = pts + 15ull * i;
     uint64_t l[5];
     fe X, Y, Z;
@@ -121,44 +134,46 @@ __global__ void __launch_bounds__(256) k_fixed_prep(const uint32_t *__restrict__
     op[1] = make_uint4(k[4], k[5], k[6], k[7]);
 }
 
-// signed 5-bit digit of window w of a canonical scalar (< 2^255)
-__device__ __forceinline__ int fixed_digit(const uint32_t *__restrict__ s, int w)
+// signed WB-bit digit of window w of a canonical scalar (< 2^255), in [-2^(WB-1), 2^(WB-1)]
+template <int WB> __device__ __forceinline__ int fixed_digit(const uint32_t *__restrict__ s, int w)
 {
+    constexpr uint32_t mask = (2u << WB) - 1; // the window's WB bits and the bit below it
     uint32_t v;
     if (w == 0)
-        v = (s[0] << 1) & 63u;
+        v = (s[0] << 1) & mask;
     else
     {
-        const int bit = 5 * w - 1, wi = bit >> 5, off = bit & 31;
+        const int bit = WB * w - 1, wi = bit >> 5, off = bit & 31;
         const uint64_t lo = s[wi];
         const uint64_t hi = wi + 1 < 8 ? s[wi + 1] : 0u;
-        v = (uint32_t)(((lo | (hi << 32)) >> off) & 63u);
+        v = (uint32_t)(((lo | (hi << 32)) >> off) & mask);
     }
     const int raw = (int)(v >> 1), cin = (int)(v & 1u);
-    return raw + cin - ((raw >> 4) << 5);
+    return raw + cin - ((raw >> (WB - 1)) << WB);
 }
 
-// One wavefront per (MSM k, window w): S[k * 52 + w] = sum_i T_i[d_kiw].
-template <class C>
+// One wavefront per (MSM k, window w): S[k * windows + w] = sum_i T_i[d_kiw].
+template <class C, int WB>
 __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__restrict__ tables,
                                                           const uint32_t *__restrict__ scal, uint32_t n,
                                                           xyzz *__restrict__ S)
 {
     using F = typename C::F;
+    constexpr int WINDOWS = FixedW<WB>::windows, ENTRIES = FixedW<WB>::entries;
     __shared__ xyzz sh[64];
-    const uint32_t k = blockIdx.x / FIXED_WINDOWS;
-    const int w = (int)(blockIdx.x % FIXED_WINDOWS);
+    const uint32_t k = blockIdx.x / WINDOWS;
+    const int w = (int)(blockIdx.x % WINDOWS);
     const uint32_t *sk = scal + (size_t)k * n * 8;
     xyzz acc;
     bool acc_id = true;
     xyzz_identity<F>(acc);
     for (uint32_t i = threadIdx.x; i < n; i += 64)
     {
-        const int d = fixed_digit(sk + 8ull * i, w);
+        const int d = fixed_digit<WB>(sk + 8ull * i, w);
         if (d == 0)
             continue;
         aff q;
-        load_aff(q, tables, i * FIXED_ENTRIES + (uint32_t)((d < 0 ? -d : d) - 1));
+        load_aff(q, tables, i * ENTRIES + (uint32_t)((d < 0 ? -d : d) - 1));
         if (fe_is_zero<F>(q.x) && fe_is_zero<F>(q.y))
             continue; // identity base
         if (d < 0)
@@ -189,24 +204,25 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
     }
 }
 
-// One quad of lanes per MSM: Horner over the 52 window sums (5 shared doublings per window),
+// One quad of lanes per MSM: Horner over the window sums (WB shared doublings per window),
 // then the 32-byte encoding and the reference struct (affine, Z = 1; identity 1:1:0).
-template <class C>
+template <class C, int WB>
 __global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S, uint32_t K,
                                                      uint32_t *__restrict__ out32, uint64_t *__restrict__ outjac)
 {
     using F = typename C::F;
+    constexpr int WINDOWS = FixedW<WB>::windows;
     const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (k >= K)
         return;
-    const xyzz *sk = S + (size_t)k * FIXED_WINDOWS;
+    const xyzz *sk = S + (size_t)k * WINDOWS;
     xyzz s;
     jac acc, t;
-    load_xyzz(s, sk + FIXED_WINDOWS - 1);
+    load_xyzz(s, sk + WINDOWS - 1);
     xyzz_to_jac_q<F>(acc, s);
-    for (int w = FIXED_WINDOWS - 2; w >= 0; w--)
+    for (int w = WINDOWS - 2; w >= 0; w--)
     {
-        jac_dbl_n_q<F>(acc, 5);
+        jac_dbl_n_q<F>(acc, WB);
         load_xyzz(s, sk + w);
         xyzz_to_jac_q<F>(t, s);
         jac_add_q<F>(acc, acc, t);

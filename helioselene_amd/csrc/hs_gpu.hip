@@ -906,6 +906,7 @@ struct hs_gpu_fixed
     int curve = 0;
     uint32_t n = 0;
     DevBuf tables, points; // n x 16 canonical affine entries; n reference Jacobian structs
+    bool wide = false;     // tables hold n x 128 entries (8-bit windows) instead
     DevBuf scal, sums;     // per-call workspace: prepared scalars, window sums
     hs_gpu_batch *pip = nullptr; // cached Pippenger plan (method HS_GPU_FIXED_PIPPENGER)
     const void *pip_s = nullptr;
@@ -946,30 +947,51 @@ int fixed_alloc(hs_gpu_fixed **out, int curve, size_t n)
     return 0;
 }
 
+// Sets of at most this many bases built from points get 8-bit-window tables (8 KiB per base).
+constexpr size_t FIXED_WIDE_MAX_BASES = (size_t)1 << 20;
+
+// Tables of a set from its points.  With d_ref_out (the reference-format precompute) they are the
+// width-5 tables; otherwise sets up to FIXED_WIDE_MAX_BASES bases get width-8 tables.
 template <class C> int fixed_build_tables(hs_gpu_fixed &f, uint64_t *d_ref_out, hipStream_t st)
 {
-    const uint32_t t = f.n * hs::FIXED_ENTRIES;
-    hipLaunchKernelGGL(hs::k_fixed_tables<C>, dim3((t + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
-                       f.tables.as<uint32_t>(), d_ref_out);
+    if (!d_ref_out && f.n <= FIXED_WIDE_MAX_BASES)
+    {
+        constexpr int E = hs::FixedW<8>::entries;
+        int r;
+        if ((r = f.tables.reserve((size_t)f.n * E * 64)))
+            return r;
+        const uint32_t t = f.n * (uint32_t)E;
+        hipLaunchKernelGGL((hs::k_fixed_tables<C, E>), dim3((t + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(),
+                           f.n, f.tables.as<uint32_t>(), (uint64_t *)nullptr);
+        f.wide = true;
+    }
+    else
+    {
+        const uint32_t t = f.n * hs::FIXED_ENTRIES;
+        hipLaunchKernelGGL((hs::k_fixed_tables<C, hs::FIXED_ENTRIES>), dim3((t + 255) / 256), dim3(256), 0, st,
+                           f.points.as<uint64_t>(), f.n, f.tables.as<uint32_t>(), d_ref_out);
+        f.wide = false;
+    }
     HS_CHECK(hipGetLastError());
     return 0;
 }
 
-// count MSMs over the set; comb method (width-5 tables, shared doublings)
-template <class C>
+// count MSMs over the set; comb method (width-WB tables, shared doublings)
+template <class C, int WB>
 int fixed_comb_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, unsigned flags, void *d_out32,
                        void *d_outjac, hipStream_t st)
 {
+    constexpr int W = hs::FixedW<WB>::windows;
     int r;
     const size_t rows = count * f.n;
-    if ((r = f.scal.reserve(rows * 32)) || (r = f.sums.reserve(count * hs::FIXED_WINDOWS * sizeof(hs::xyzz))))
+    if ((r = f.scal.reserve(rows * 32)) || (r = f.sums.reserve(count * W * sizeof(hs::xyzz))))
         return r;
     const uint32_t compat = (!(flags & HS_GPU_EXACT) && ref_window(f.n) == 8) ? 1u : 0u;
     hipLaunchKernelGGL(hs::k_fixed_prep<C>, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st,
                        (const uint32_t *)d_scalars, rows, f.scal.as<uint32_t>(), compat);
-    hipLaunchKernelGGL(hs::k_fixed_window_sums<C>, dim3((unsigned)(count * hs::FIXED_WINDOWS)), dim3(64), 0, st,
+    hipLaunchKernelGGL((hs::k_fixed_window_sums<C, WB>), dim3((unsigned)(count * W)), dim3(64), 0, st,
                        f.tables.as<uint32_t>(), f.scal.as<uint32_t>(), f.n, f.sums.as<hs::xyzz>());
-    hipLaunchKernelGGL(hs::k_fixed_horner<C>, dim3((unsigned)((4 * count + 63) / 64)), dim3(64), 0, st,
+    hipLaunchKernelGGL((hs::k_fixed_horner<C, WB>), dim3((unsigned)((4 * count + 63) / 64)), dim3(64), 0, st,
                        f.sums.as<hs::xyzz>(), (uint32_t)count, (uint32_t *)d_out32, (uint64_t *)d_outjac);
     HS_CHECK(hipGetLastError());
     return 0;
@@ -1021,9 +1043,13 @@ int fixed_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, unsigned
         return fail(HS_GPU_EINVAL, "unknown fixed-base method");
     if (count * hs::FIXED_WINDOWS >= ((size_t)1 << 31))
         return fail(HS_GPU_EINVAL, "too many MSMs in one fixed-base call");
+    if (f.wide)
+        return f.curve == HS_GPU_HELIOS
+                   ? fixed_comb_enqueue<hs::HeliosC, 8>(f, d_scalars, count, flags, d_out32, d_outjac, st)
+                   : fixed_comb_enqueue<hs::SeleneC, 8>(f, d_scalars, count, flags, d_out32, d_outjac, st);
     return f.curve == HS_GPU_HELIOS
-               ? fixed_comb_enqueue<hs::HeliosC>(f, d_scalars, count, flags, d_out32, d_outjac, st)
-               : fixed_comb_enqueue<hs::SeleneC>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+               ? fixed_comb_enqueue<hs::HeliosC, 5>(f, d_scalars, count, flags, d_out32, d_outjac, st)
+               : fixed_comb_enqueue<hs::SeleneC, 5>(f, d_scalars, count, flags, d_out32, d_outjac, st);
 }
 
 // Host-buffer round trip of `count` MSMs over f (synchronous on the library stream).
