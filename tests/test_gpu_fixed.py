@@ -1,4 +1,4 @@
-"""GPU parity tests for Pedersen commitments and the fixed-base (width-5 table) MSM.
+"""GPU parity tests for Pedersen commitments and the fixed-base comb MSM (width-5 and width-8 tables).
 
 Checked against the reference's golden pedersen_commit vectors, the oracle's restatements of
 helios_pedersen_commit / helios_msm_fixed (oracle/hs_oracle.c), the oracle's scalar multiple for
