@@ -221,11 +221,13 @@ def run_pedersen(hs, torch, args, count=1024, m=4096):
             pts = torch.empty((m + 1, 15), dtype=torch.int64, device=dev)
             hs.map_to_curve_device(c, u.data_ptr(), m + 1, pts.data_ptr(), None, sh)
             torch.cuda.synchronize()
+            t0 = time.perf_counter()
             fs = hs.FixedSet(c, d_points=pts.data_ptr(), n=m + 1, stream=sh)
+            res[f"{['helios', 'selene'][c]}_table_build_ms"] = (time.perf_counter() - t0) * 1e3
             s = rng.integers(0, 256, size=(count, m + 1, 32), dtype=np.uint8)
             s[:, :, 31] &= 0x7F
             ds = torch.from_numpy(s).to(dev)
-            for method, name in ((hs.FIXED_COMB, "comb_w8"), (hs.FIXED_PIPPENGER, "pippenger")):
+            for method, name in ((hs.FIXED_COMB, "comb"), (hs.FIXED_PIPPENGER, "pippenger")):
                 out = torch.zeros((count, 8), dtype=torch.int32, device=dev)
                 fs.msm_device(ds.data_ptr(), count, out.data_ptr(), None, 0, method, sh)
                 torch.cuda.synchronize()
@@ -239,10 +241,10 @@ def run_pedersen(hs, torch, args, count=1024, m=4096):
                 res[f"{['helios', 'selene'][c]}_{name}_ms"] = tm * 1e3
                 res[f"{['helios', 'selene'][c]}_{name}_points_per_s"] = count * (m + 1) / tm
                 outs[(c, name)] = out.cpu().numpy()
-            res[f"{['helios', 'selene'][c]}_methods_agree"] = bool((outs[(c, "comb_w8")] == outs[(c, "pippenger")]).all())
+            res[f"{['helios', 'selene'][c]}_methods_agree"] = bool((outs[(c, "comb")] == outs[(c, "pippenger")]).all())
             fs.close()
             del ds
-    best = sum(min(res[f"{cv}_comb_w8_ms"], res[f"{cv}_pippenger_ms"]) for cv in ("helios", "selene"))
+    best = sum(min(res[f"{cv}_comb_ms"], res[f"{cv}_pippenger_ms"]) for cv in ("helios", "selene"))
     res["points_per_s_best_method"] = 2 * count * (m + 1) / (best * 1e-3)
     return res
 

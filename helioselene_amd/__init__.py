@@ -377,8 +377,8 @@ def msm_fixed(curve, scalars, tables):
 
 
 class FixedSet:
-    """A resident fixed-base set (bases and their comb tables kept in HBM: width 8 up to 2^20
-    bases when built from points, width 5 otherwise)."""
+    """A resident fixed-base set (bases and their comb tables kept in HBM: windows of up to 16 bits,
+    sized to a 9 GiB table budget)."""
 
     def __init__(self, curve, points=None, d_points=None, n=None, stream=None):
         self.curve = CURVES[curve]

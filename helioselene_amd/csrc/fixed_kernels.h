@@ -1,4 +1,4 @@
-// Fixed-base multi-scalar multiplication over resident width-5 tables.
+// Fixed-base multi-scalar multiplication over resident tables of multiples.
 //
 // Restates the algorithm of helios_msm_fixed (helios/include/helios_msm_fixed.h:60-155) and its
 // tables (helios_scalarmult_fixed_precompute, helios/include/helios_scalarmult_fixed.h:54-65):
@@ -14,9 +14,9 @@
 // recoding differs from helios_scalar_recode_signed5 digit by digit but represents the same
 // integer, and group arithmetic here is exact, so every result is the same point.
 //
-// Resident sets built from points use the same scheme with 8-bit windows (128 multiples per
-// base, 32 windows, 8 doublings per window); the drop-in msm_fixed keeps the reference's width-5
-// tables.
+// Resident sets built from points use the same scheme with wider windows (up to 16 bits: 2^15
+// multiples per base, 16 windows, 16 doublings per window), chosen by the table memory they need;
+// the drop-in msm_fixed keeps the reference's width-5 tables.
 //
 // Table entries are canonical affine (x, y) as 16 little-endian words; an all-zero entry is the
 // identity (the table of an identity base).
@@ -31,17 +31,50 @@ constexpr int FIXED_ENTRIES = 16;
 
 // Table geometry for window width WB: 2^(WB-1) multiples per base, ceil(256 / WB) signed windows.
 // WB = 5 is the reference's layout (the drop-in tables); resident sets built from points use
-// WB = 8 (32 additions per base and MSM instead of 52, 8 KiB of table per base).
+// WB = 8..16 (32..16 additions per base and MSM instead of 52; 8 KiB..2 MiB of table per base).
 template <int WB> struct FixedW
 {
     static constexpr int entries = 1 << (WB - 1);
     static constexpr int windows = (256 + WB - 1) / WB;
 };
 
-// Tables from reference Jacobian structs: thread (i, k) computes (k+1) * P_i.  Optionally also
-// writes the reference's helios_affine layout (x[5], y[5] radix-2^51, canonical).
+// Bases from reference Jacobian structs to canonical affine entries (all-zero for the identity),
+// once per base, ahead of k_fixed_tables.
+template <class C>
+__global__ void __launch_bounds__(256) k_fixed_bases(const uint64_t *__restrict__ pts, uint32_t n,
+                                                     uint32_t *__restrict__ base_aff)
+{
+    using F = typename C::F;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t *p = pts + 15ull * i;
+    uint64_t l[5];
+    jac J;
+    for (int j = 0; j < 5; j++)
+        l[j] = p[j];
+    fe_from_r51<F>(J.X, l);
+    for (int j = 0; j < 5; j++)
+        l[j] = p[5 + j];
+    fe_from_r51<F>(J.Y, l);
+    for (int j = 0; j < 5; j++)
+        l[j] = p[10 + j];
+    fe_from_r51<F>(J.Z, l);
+    aff a;
+    if (fe_is_zero<F>(J.Z))
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+    }
+    else
+        jac_to_aff<F>(a, J);
+    store_aff<F>(base_aff, i, a);
+}
+
+// Tables from the affine bases: thread (i, k) computes (k+1) * P_i.  Optionally also writes the
+// reference's helios_affine layout (x[5], y[5] radix-2^51, canonical).
 template <class C, int ENTRIES>
-__global__ void __launch_bounds__(256) k_fixed_tables(const uint64_t *__restrict__ pts, uint32_t n,
+__global__ void __launch_bounds__(256) k_fixed_tables(const uint32_t *__restrict__ base_aff, uint32_t n,
                                                       uint32_t *__restrict__ tables, uint64_t *__restrict__ ref_out)
 {
     using F = typename C::F;
@@ -49,28 +82,11 @@ __global__ void __launch_bounds__(256) k_fixed_tables(const uint64_t *__restrict
     if (t >= n * ENTRIES)
         return;
     const uint32_t i = t / ENTRIES, k = t % ENTRIES;
-    const uint64_t *p = pts + 15ull * i;
-    uint64_t l[5];
-    fe X, Y, Z;
-    for (int j = 0; j < 5; j++)
-        l[j] = p[j];
-    fe_from_r51<F>(X, l);
-    for (int j = 0; j < 5; j++)
-        l[j] = p[5 + j];
-    fe_from_r51<F>(Y, l);
-    for (int j = 0; j < 5; j++)
-        l[j] = p[10 + j];
-    fe_from_r51<F>(Z, l);
-    aff a;
-    bool pt = !fe_is_zero<F>(Z);
+    aff base, a;
+    load_aff(base, base_aff, i);
+    bool pt = !(fe_is_zero<F>(base.x) && fe_is_zero<F>(base.y));
     if (pt)
     {
-        jac J;
-        J.X = X;
-        J.Y = Y;
-        J.Z = Z;
-        aff base;
-        jac_to_aff<F>(base, J);
         xyzz b, m;
         xyzz_from_aff<F>(b, base);
         xyzz_mul_small<F>(m, b, k + 1);
@@ -172,12 +188,25 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
         const int d = fixed_digit<WB>(sk + 8ull * i, w);
         if (d == 0)
             continue;
+        const uint4 *p = reinterpret_cast<const uint4 *>(tables + 16ull * (i * ENTRIES + (uint32_t)((d < 0 ? -d : d) - 1)));
+        const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+        const uint32_t xw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        const uint32_t yw[8] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+        uint32_t any = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            any |= xw[j] | yw[j];
+        if (any == 0)
+            continue; // identity base (all-zero entry)
         aff q;
-        load_aff(q, tables, i * ENTRIES + (uint32_t)((d < 0 ? -d : d) - 1));
-        if (fe_is_zero<F>(q.x) && fe_is_zero<F>(q.y))
-            continue; // identity base
-        if (d < 0)
-            fe_neg<F>(q.y, q.y);
+        fe_from_words(q.x, xw);
+        fe_from_words(q.y, yw);
+        // a negative digit adds -T: y -> 65 m - y limb by limb (lazy limbs below 2^30, which
+        // xyzz_madd accepts for the base's y)
+        const uint32_t keep = d < 0 ? 0u : 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < NL; j++)
+            q.y.v[j] = (q.y.v[j] & keep) | ((F::bias2(j) - q.y.v[j]) & ~keep);
         xyzz_madd<F>(acc, acc_id, q);
     }
     if (acc_id)

@@ -906,7 +906,7 @@ struct hs_gpu_fixed
     int curve = 0;
     uint32_t n = 0;
     DevBuf tables, points; // n x 16 canonical affine entries; n reference Jacobian structs
-    bool wide = false;     // tables hold n x 128 entries (8-bit windows) instead
+    int wb = 5;            // window width of the tables: n x 2^(wb-1) entries
     DevBuf scal, sums;     // per-call workspace: prepared scalars, window sums
     hs_gpu_batch *pip = nullptr; // cached Pippenger plan (method HS_GPU_FIXED_PIPPENGER)
     const void *pip_s = nullptr;
@@ -947,33 +947,59 @@ int fixed_alloc(hs_gpu_fixed **out, int curve, size_t n)
     return 0;
 }
 
-// Sets of at most this many bases built from points get 8-bit-window tables (8 KiB per base).
-constexpr size_t FIXED_WIDE_MAX_BASES = (size_t)1 << 20;
+// Window width of the tables of a set built from points: the widest of 16, 14, 13, 12, 10 and 8
+// bits whose tables (n x 2^(wb-1) entries of 64 B) fit FIXED_TABLE_BUDGET, else 5.  Wider windows
+// mean fewer additions per base and MSM (16 instead of 32 at width 16); the table gathers are
+// random 64-byte reads that HBM serves well.  2^12 + 1 bases get width 16 (8 GiB), 2^20 bases
+// width 8.  Entry indices stay below 2^32.
+constexpr size_t FIXED_TABLE_BUDGET = (size_t)9 << 30;
+
+int fixed_width(size_t n)
+{
+    for (int wb : {16, 14, 13, 12, 10, 8})
+        if (n * ((size_t)64 << (wb - 1)) <= FIXED_TABLE_BUDGET)
+            return wb;
+    return 5;
+}
+
+template <class C, int WB> int fixed_build_tables_w(hs_gpu_fixed &f, uint64_t *d_ref_out, hipStream_t st)
+{
+    constexpr int E = hs::FixedW<WB>::entries;
+    int r;
+    // the affine bases are staged in the per-call scalar workspace
+    if ((r = f.tables.reserve((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * 64)))
+        return r;
+    hipLaunchKernelGGL(hs::k_fixed_bases<C>, dim3((f.n + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
+                       f.scal.as<uint32_t>());
+    const uint32_t t = f.n * (uint32_t)E;
+    hipLaunchKernelGGL((hs::k_fixed_tables<C, E>), dim3((t + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
+                       f.n, f.tables.as<uint32_t>(), d_ref_out);
+    HS_CHECK(hipGetLastError());
+    f.wb = WB;
+    return 0;
+}
 
 // Tables of a set from its points.  With d_ref_out (the reference-format precompute) they are the
-// width-5 tables; otherwise sets up to FIXED_WIDE_MAX_BASES bases get width-8 tables.
+// width-5 tables; otherwise the width is fixed_width(n).
 template <class C> int fixed_build_tables(hs_gpu_fixed &f, uint64_t *d_ref_out, hipStream_t st)
 {
-    if (!d_ref_out && f.n <= FIXED_WIDE_MAX_BASES)
+    switch (d_ref_out ? 5 : fixed_width(f.n))
     {
-        constexpr int E = hs::FixedW<8>::entries;
-        int r;
-        if ((r = f.tables.reserve((size_t)f.n * E * 64)))
-            return r;
-        const uint32_t t = f.n * (uint32_t)E;
-        hipLaunchKernelGGL((hs::k_fixed_tables<C, E>), dim3((t + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(),
-                           f.n, f.tables.as<uint32_t>(), (uint64_t *)nullptr);
-        f.wide = true;
+    case 16:
+        return fixed_build_tables_w<C, 16>(f, nullptr, st);
+    case 14:
+        return fixed_build_tables_w<C, 14>(f, nullptr, st);
+    case 13:
+        return fixed_build_tables_w<C, 13>(f, nullptr, st);
+    case 12:
+        return fixed_build_tables_w<C, 12>(f, nullptr, st);
+    case 10:
+        return fixed_build_tables_w<C, 10>(f, nullptr, st);
+    case 8:
+        return fixed_build_tables_w<C, 8>(f, nullptr, st);
+    default:
+        return fixed_build_tables_w<C, 5>(f, d_ref_out, st);
     }
-    else
-    {
-        const uint32_t t = f.n * hs::FIXED_ENTRIES;
-        hipLaunchKernelGGL((hs::k_fixed_tables<C, hs::FIXED_ENTRIES>), dim3((t + 255) / 256), dim3(256), 0, st,
-                           f.points.as<uint64_t>(), f.n, f.tables.as<uint32_t>(), d_ref_out);
-        f.wide = false;
-    }
-    HS_CHECK(hipGetLastError());
-    return 0;
 }
 
 // count MSMs over the set; comb method (width-WB tables, shared doublings)
@@ -995,6 +1021,29 @@ int fixed_comb_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, uns
                        f.sums.as<hs::xyzz>(), (uint32_t)count, (uint32_t *)d_out32, (uint64_t *)d_outjac);
     HS_CHECK(hipGetLastError());
     return 0;
+}
+
+template <class C>
+int fixed_comb_dispatch(hs_gpu_fixed &f, const void *d_scalars, size_t count, unsigned flags, void *d_out32,
+                        void *d_outjac, hipStream_t st)
+{
+    switch (f.wb)
+    {
+    case 16:
+        return fixed_comb_enqueue<C, 16>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    case 14:
+        return fixed_comb_enqueue<C, 14>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    case 13:
+        return fixed_comb_enqueue<C, 13>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    case 12:
+        return fixed_comb_enqueue<C, 12>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    case 10:
+        return fixed_comb_enqueue<C, 10>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    case 8:
+        return fixed_comb_enqueue<C, 8>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    default:
+        return fixed_comb_enqueue<C, 5>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    }
 }
 
 // Pippenger method: one batched plan whose MSMs all read the resident bases
@@ -1043,13 +1092,8 @@ int fixed_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, unsigned
         return fail(HS_GPU_EINVAL, "unknown fixed-base method");
     if (count * hs::FIXED_WINDOWS >= ((size_t)1 << 31))
         return fail(HS_GPU_EINVAL, "too many MSMs in one fixed-base call");
-    if (f.wide)
-        return f.curve == HS_GPU_HELIOS
-                   ? fixed_comb_enqueue<hs::HeliosC, 8>(f, d_scalars, count, flags, d_out32, d_outjac, st)
-                   : fixed_comb_enqueue<hs::SeleneC, 8>(f, d_scalars, count, flags, d_out32, d_outjac, st);
-    return f.curve == HS_GPU_HELIOS
-               ? fixed_comb_enqueue<hs::HeliosC, 5>(f, d_scalars, count, flags, d_out32, d_outjac, st)
-               : fixed_comb_enqueue<hs::SeleneC, 5>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    return f.curve == HS_GPU_HELIOS ? fixed_comb_dispatch<hs::HeliosC>(f, d_scalars, count, flags, d_out32, d_outjac, st)
+                                    : fixed_comb_dispatch<hs::SeleneC>(f, d_scalars, count, flags, d_out32, d_outjac, st);
 }
 
 // Host-buffer round trip of `count` MSMs over f (synchronous on the library stream).
