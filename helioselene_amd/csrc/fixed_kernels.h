@@ -14,9 +14,10 @@
 // recoding differs from helios_scalar_recode_signed5 digit by digit but represents the same
 // integer, and group arithmetic here is exact, so every result is the same point.
 //
-// Resident sets built from points use the same scheme with wider windows (up to 16 bits: 2^15
-// multiples per base, 16 windows, 16 doublings per window), chosen by the table memory they need;
-// the drop-in msm_fixed keeps the reference's width-5 tables.
+// Resident sets built from points use a 16-tooth signed-binary comb (FixedW<FIXED_COMB16>: 16
+// columns, one shared doubling per column) or, when its tables do not fit, the same windowed
+// scheme with wider windows (up to 14 bits), chosen by the table memory they need; the drop-in
+// msm_fixed keeps the reference's width-5 tables.
 //
 // Table entries are canonical affine (x, y) as 16 little-endian words; an all-zero entry is the
 // identity (the table of an identity base).
@@ -29,13 +30,29 @@ namespace hs
 constexpr int FIXED_WINDOWS = 52;
 constexpr int FIXED_ENTRIES = 16;
 
-// Table geometry for window width WB: 2^(WB-1) multiples per base, ceil(256 / WB) signed windows.
-// WB = 5 is the reference's layout (the drop-in tables); resident sets built from points use
-// WB = 8..16 (32..16 additions per base and MSM instead of 52; 8 KiB..2 MiB of table per base).
+// Table geometry for window width WB: 2^(WB-1) multiples per base, ceil(256 / WB) signed windows
+// of WB doublings each.  WB = 5 is the reference's layout (the drop-in tables); resident sets built
+// from points use WB = 8..14 (32..19 additions per base and MSM instead of 52).
 template <int WB> struct FixedW
 {
     static constexpr int entries = 1 << (WB - 1);
     static constexpr int windows = (256 + WB - 1) / WB;
+    static constexpr int step = WB;
+};
+
+// The signed-binary comb with 16 teeth (width code 0).  A scalar s is made odd (s' = s, or
+// s + order) and written as s' = sum_{i<256} b_i 2^i with every b_i = +-1.  Digit i = 16 j + r is
+// tooth j of column r, so s' P = sum_r 2^r sum_j b_{16j+r} Q_j with Q_j = 2^(16 j) P.  Entry e of
+// a base is Q_15 + sum_{j<15} (bit j of e ? Q_j : -Q_j): 2^15 entries (2 MiB per base), one
+// column = one table lookup per base, and a pattern with b_{16*15+r} = -1 uses the entry of its
+// negation, negated.  16 additions per base and MSM, like 16-bit windows, but only 15 shared
+// doublings per MSM instead of 240.
+constexpr int FIXED_COMB16 = 0;
+template <> struct FixedW<FIXED_COMB16>
+{
+    static constexpr int entries = 1 << 15;
+    static constexpr int windows = 16;
+    static constexpr int step = 1;
 };
 
 // Bases from reference Jacobian structs to canonical affine entries (all-zero for the identity),
@@ -106,6 +123,74 @@ __global__ void __launch_bounds__(256) k_fixed_tables(const uint32_t *__restrict
     }
 }
 
+// Comb tables, step 1: the tooth bases Q_j = 2^(16 j) P_i as canonical affine entries (all-zero
+// for the identity), one thread per (base i, tooth j).
+template <class C>
+__global__ void __launch_bounds__(256) k_comb_teeth(const uint32_t *__restrict__ base_aff, uint32_t n,
+                                                    uint32_t *__restrict__ teeth)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 16)
+        return;
+    const uint32_t i = t >> 4, j = t & 15;
+    aff b, a;
+    load_aff(b, base_aff, i);
+    bool pt = !(fe_is_zero<F>(b.x) && fe_is_zero<F>(b.y));
+    if (pt)
+    {
+        xyzz q;
+        xyzz_from_aff<F>(q, b);
+        for (uint32_t d = 0; d < 16 * j; d++)
+            xyzz_dbl<F>(q, q);
+        pt = xyzz_to_aff<F>(a, q);
+    }
+    if (!pt)
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+    }
+    store_aff<F>(teeth, t, a);
+}
+
+// Comb tables, step 2: entry e of base i is Q_15 + sum_{j<15} (bit j of e ? Q_j : -Q_j), one
+// thread per entry.  No entry of a non-identity base is the identity: |2^240 + sum_{j<15} +-2^(16 j)|
+// lies in (0, 2^241), below the group order.
+template <class C>
+__global__ void __launch_bounds__(256) k_comb_tables(const uint32_t *__restrict__ teeth, uint32_t n,
+                                                     uint32_t *__restrict__ tables)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (n << 15))
+        return;
+    const uint32_t i = t >> 15, e = t & 0x7fffu;
+    aff q, a;
+    load_aff(q, teeth, 16 * i + 15);
+    bool pt = !(fe_is_zero<F>(q.x) && fe_is_zero<F>(q.y));
+    if (pt)
+    {
+        xyzz acc;
+        bool acc_id = true;
+        xyzz_identity<F>(acc);
+        xyzz_madd<F>(acc, acc_id, q);
+        for (int j = 0; j < 15; j++)
+        {
+            load_aff(q, teeth, 16 * i + j);
+            if (!((e >> j) & 1u))
+                fe_neg<F>(q.y, q.y);
+            xyzz_madd<F>(acc, acc_id, q);
+        }
+        pt = !acc_id && xyzz_to_aff<F>(a, acc);
+    }
+    if (!pt)
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+    }
+    store_aff<F>(tables, t, a);
+}
+
 // Tables given in the reference's helios_affine layout (possibly lazy limbs): canonicalise.
 // (0, 0) stays the all-zero identity entry.
 template <class C>
@@ -131,11 +216,47 @@ __global__ void __launch_bounds__(256) k_fixed_tables_from_ref(const uint64_t *_
     store_aff<F>(tables, t, a);
 }
 
+// Canonical scalar (< order < 2^255) -> the 16 column patterns of its signed-binary comb form
+// (FixedW<FIXED_COMB16>): s' = s, or s + order when s is even, so s' is odd and below 2^256; then
+// s' = sum_i (2 u_i - 1) 2^i for u = floor(s' / 2) + 2^255.  Pattern r (16 bits, halfword r of the
+// output) holds bit 16 j + r of u at bit j.
+template <class O> HS_HD void comb_recode(uint32_t k[8])
+{
+    if (!(k[0] & 1u))
+    {
+        uint64_t t = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+        {
+            t = (uint64_t)k[i] + O::mw(i) + (t >> 32);
+            k[i] = (uint32_t)t;
+        }
+    }
+    uint32_t u[8];
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+        u[i] = (k[i] >> 1) | (k[i + 1] << 31);
+    u[7] = (k[7] >> 1) | 0x80000000u;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        k[i] = 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+    {
+        uint32_t pr = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            pr |= ((u[j >> 1] >> (16 * (j & 1) + r)) & 1u) << j;
+        k[r >> 1] |= pr << (16 * (r & 1));
+    }
+}
+
 // Scalars -> canonical words mod the group order, with the reference's window-8 substitution
-// when `w8_compat` is set (each row then matches helios_msm_vartime over the same bases).
+// when `w8_compat` is set (each row then matches helios_msm_vartime over the same bases); with
+// `comb`, then recoded to comb column patterns.
 template <class C>
 __global__ void __launch_bounds__(256) k_fixed_prep(const uint32_t *__restrict__ in, size_t count,
-                                                    uint32_t *__restrict__ out, uint32_t w8_compat)
+                                                    uint32_t *__restrict__ out, uint32_t w8_compat, uint32_t comb)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count)
@@ -145,6 +266,8 @@ __global__ void __launch_bounds__(256) k_fixed_prep(const uint32_t *__restrict__
     const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     uint32_t k[8];
     scalar_prepare<typename C::O>(k, s, w8_compat != 0);
+    if (comb)
+        comb_recode<typename C::O>(k);
     uint4 *op = reinterpret_cast<uint4 *>(out + 8 * t);
     op[0] = make_uint4(k[0], k[1], k[2], k[3]);
     op[1] = make_uint4(k[4], k[5], k[6], k[7]);
@@ -166,6 +289,13 @@ template <int WB> __device__ __forceinline__ int fixed_digit(const uint32_t *__r
     }
     const int raw = (int)(v >> 1), cin = (int)(v & 1u);
     return raw + cin - ((raw >> (WB - 1)) << WB);
+}
+
+// Comb column r: +-(entry + 1) of the pattern, never 0.
+template <> __device__ __forceinline__ int fixed_digit<FIXED_COMB16>(const uint32_t *__restrict__ s, int r)
+{
+    const uint32_t p = (s[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+    return (p & 0x8000u) ? (int)(p & 0x7fffu) + 1 : -(int)((~p & 0x7fffu) + 1);
 }
 
 // One wavefront per (MSM k, window w): S[k * windows + w] = sum_i T_i[d_kiw].
@@ -233,7 +363,7 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
     }
 }
 
-// One quad of lanes per MSM: Horner over the window sums (WB shared doublings per window),
+// One quad of lanes per MSM: Horner over the window sums (step shared doublings per window),
 // then the 32-byte encoding and the reference struct (affine, Z = 1; identity 1:1:0).
 template <class C, int WB>
 __global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S, uint32_t K,
@@ -251,7 +381,7 @@ __global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S,
     xyzz_to_jac_q<F>(acc, s);
     for (int w = WINDOWS - 2; w >= 0; w--)
     {
-        jac_dbl_n_q<F>(acc, WB);
+        jac_dbl_n_q<F>(acc, FixedW<WB>::step);
         load_xyzz(s, sk + w);
         xyzz_to_jac_q<F>(t, s);
         jac_add_q<F>(acc, acc, t);

@@ -906,7 +906,7 @@ struct hs_gpu_fixed
     int curve = 0;
     uint32_t n = 0;
     DevBuf tables, points; // n x 16 canonical affine entries; n reference Jacobian structs
-    int wb = 5;            // window width of the tables: n x 2^(wb-1) entries
+    int wb = 5;            // table layout: window width (n x 2^(wb-1) entries) or hs::FIXED_COMB16
     DevBuf scal, sums;     // per-call workspace: prepared scalars, window sums
     hs_gpu_batch *pip = nullptr; // cached Pippenger plan (method HS_GPU_FIXED_PIPPENGER)
     const void *pip_s = nullptr;
@@ -947,19 +947,43 @@ int fixed_alloc(hs_gpu_fixed **out, int curve, size_t n)
     return 0;
 }
 
-// Window width of the tables of a set built from points: the widest of 16, 14, 13, 12, 10 and 8
-// bits whose tables (n x 2^(wb-1) entries of 64 B) fit FIXED_TABLE_BUDGET, else 5.  Wider windows
-// mean fewer additions per base and MSM (16 instead of 32 at width 16); the table gathers are
-// random 64-byte reads that HBM serves well.  2^12 + 1 bases get width 16 (8 GiB), 2^20 bases
-// width 8.  Entry indices stay below 2^32.
+// Tables of a set built from points: the 16-tooth comb (2^15 entries per base, width code
+// hs::FIXED_COMB16) when its tables (2 MiB per base) fit FIXED_TABLE_BUDGET, else the widest window
+// of 14, 13, 12, 10 and 8 bits whose 2^(w-1) entries of 64 B per base fit, else 5 bits.  Fewer
+// windows mean fewer additions per base and MSM (16 for the comb instead of 52); the table
+// gathers are random 64-byte reads that HBM serves well.  2^12 + 1 bases get the comb (8 GiB),
+// 2^20 bases width 8.  Entry indices stay below 2^32.
 constexpr size_t FIXED_TABLE_BUDGET = (size_t)9 << 30;
 
 int fixed_width(size_t n)
 {
-    for (int wb : {16, 14, 13, 12, 10, 8})
+    if (n * ((size_t)64 << 15) <= FIXED_TABLE_BUDGET)
+        return hs::FIXED_COMB16;
+    for (int wb : {14, 13, 12, 10, 8})
         if (n * ((size_t)64 << (wb - 1)) <= FIXED_TABLE_BUDGET)
             return wb;
     return 5;
+}
+
+// Comb tables: tooth bases 2^(16 j) P_i (staged in the per-call window-sum workspace), then every
+// entry from them.
+template <class C> int fixed_build_comb(hs_gpu_fixed &f, hipStream_t st)
+{
+    constexpr int E = hs::FixedW<hs::FIXED_COMB16>::entries;
+    int r;
+    if ((r = f.tables.reserve((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * 64)) ||
+        (r = f.sums.reserve((size_t)f.n * 16 * 64)))
+        return r;
+    hipLaunchKernelGGL(hs::k_fixed_bases<C>, dim3((f.n + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
+                       f.scal.as<uint32_t>());
+    hipLaunchKernelGGL(hs::k_comb_teeth<C>, dim3((16 * f.n + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
+                       f.n, f.sums.as<uint32_t>());
+    const uint32_t t = f.n * (uint32_t)E;
+    hipLaunchKernelGGL(hs::k_comb_tables<C>, dim3((t + 255) / 256), dim3(256), 0, st, f.sums.as<uint32_t>(), f.n,
+                       f.tables.as<uint32_t>());
+    HS_CHECK(hipGetLastError());
+    f.wb = hs::FIXED_COMB16;
+    return 0;
 }
 
 template <class C, int WB> int fixed_build_tables_w(hs_gpu_fixed &f, uint64_t *d_ref_out, hipStream_t st)
@@ -985,8 +1009,8 @@ template <class C> int fixed_build_tables(hs_gpu_fixed &f, uint64_t *d_ref_out, 
 {
     switch (d_ref_out ? 5 : fixed_width(f.n))
     {
-    case 16:
-        return fixed_build_tables_w<C, 16>(f, nullptr, st);
+    case hs::FIXED_COMB16:
+        return fixed_build_comb<C>(f, st);
     case 14:
         return fixed_build_tables_w<C, 14>(f, nullptr, st);
     case 13:
@@ -1014,7 +1038,8 @@ int fixed_comb_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, uns
         return r;
     const uint32_t compat = (!(flags & HS_GPU_EXACT) && ref_window(f.n) == 8) ? 1u : 0u;
     hipLaunchKernelGGL(hs::k_fixed_prep<C>, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st,
-                       (const uint32_t *)d_scalars, rows, f.scal.as<uint32_t>(), compat);
+                       (const uint32_t *)d_scalars, rows, f.scal.as<uint32_t>(), compat,
+                       WB == hs::FIXED_COMB16 ? 1u : 0u);
     hipLaunchKernelGGL((hs::k_fixed_window_sums<C, WB>), dim3((unsigned)(count * W)), dim3(64), 0, st,
                        f.tables.as<uint32_t>(), f.scal.as<uint32_t>(), f.n, f.sums.as<hs::xyzz>());
     hipLaunchKernelGGL((hs::k_fixed_horner<C, WB>), dim3((unsigned)((4 * count + 63) / 64)), dim3(64), 0, st,
@@ -1029,8 +1054,8 @@ int fixed_comb_dispatch(hs_gpu_fixed &f, const void *d_scalars, size_t count, un
 {
     switch (f.wb)
     {
-    case 16:
-        return fixed_comb_enqueue<C, 16>(f, d_scalars, count, flags, d_out32, d_outjac, st);
+    case hs::FIXED_COMB16:
+        return fixed_comb_enqueue<C, hs::FIXED_COMB16>(f, d_scalars, count, flags, d_out32, d_outjac, st);
     case 14:
         return fixed_comb_enqueue<C, 14>(f, d_scalars, count, flags, d_out32, d_outjac, st);
     case 13:

@@ -154,13 +154,16 @@ void hs_gpu_selene_msm_fixed(hs_gpu_jacobian *result, const unsigned char *scala
                              size_t n);
 
 /* Resident fixed-base sets (no reference counterpart: the reference recomputes per call).  A set
- * keeps n bases and their tables in HBM.  Tables use the widest window width w of 16, 14, 13, 12,
- * 10 and 8 bits whose 2^(w-1) multiples per base (64 B each) fit 9 GiB, else 5 bits: 2^12 + 1
- * bases get width 16 (8 GiB), 2^20 bases width 8 (8 GiB).  Each MSM over the set takes one row
- * of n scalars (n x 32 bytes) and has the plain MSM's result semantics (HS_GPU_EXACT as there).
+ * keeps n bases and their tables in HBM, at most 9 GiB of tables per set: a signed-binary comb
+ * with 16 teeth (2^15 combinations of 2^(16 j) P per base, 2 MiB) up to 4608 bases, else the
+ * widest window width w of 14, 13, 12, 10 and 8 bits whose 2^(w-1) multiples per base (64 B
+ * each) fit, else 5 bits.  2^12 + 1 bases get the comb (8 GiB), 2^20 bases width 8 (8 GiB).  Each
+ * MSM over the set takes one row of n scalars (n x 32 bytes) and has the plain MSM's result
+ * semantics (HS_GPU_EXACT as there).
  * method:
- * HS_GPU_FIXED_COMB    the tables with shared windows (ceil(256/w) windows of w doublings;
- *                      helios_msm_fixed's algorithm at w = 5), one wavefront per (MSM, window);
+ * HS_GPU_FIXED_COMB    the tables with shared doublings (16 columns of 1 doubling for the comb,
+ *                      ceil(256/w) windows of w doublings otherwise: helios_msm_fixed's algorithm
+ *                      at w = 5), one wavefront per (MSM, column or window);
  * HS_GPU_FIXED_PIPPENGER  the batched Pippenger over the resident bases;
  * HS_GPU_FIXED_AUTO    the faster of the two for this shape.
  * A Pedersen set holds [H, G_0 .. G_{m-1}] (n = m + 1); hs_gpu_pedersen_commit_batch takes

@@ -1,4 +1,4 @@
-"""GPU parity tests for Pedersen commitments and the fixed-base comb MSM (width-5 and width-8 tables).
+"""GPU parity tests for Pedersen commitments and the fixed-base MSM (16-tooth comb and windowed tables).
 
 Checked against the reference's golden pedersen_commit vectors, the oracle's restatements of
 helios_pedersen_commit / helios_msm_fixed (oracle/hs_oracle.c), the oracle's scalar multiple for
@@ -164,3 +164,23 @@ def test_pedersen_batch_vs_compiled_reference(c):
         out = ctypes.create_string_buffer(32)
         assert f(out, rows[k][:32], pb[:32], rows[k][32:], pb[32:], m) == 0
         assert comb[k] == out.raw and pip[k] == out.raw, k
+
+
+@pytest.mark.parametrize("c", [0, 1])
+def test_fixed_set_windowed_tables(c):
+    """A set too large for the comb's 2 MiB per base under the 9 GiB budget (5000 bases: 14-bit
+    windows), with an identity base and edge scalars, against the oracle and the Pippenger method."""
+    n, K = 5000, 2
+    pb = bytearray(O.gen_sswu_bases(c, 700 + c, n, dup_frac=0.01))
+    pb[32 * 11:32 * 12] = bytes(32)
+    pts = GH.to_jac_array(c, bytes(pb), n)
+    rows = [O.gen_scalars(c, 710 + 10 * c + k, n, edge_frac=0.05) for k in range(K)]
+    S = np.frombuffer(b"".join(rows), np.uint8).reshape(K, n, 32)
+    fs = hs.FixedSet(c, pts)
+    comb, jac = fs.msm(S, flags=hs.EXACT, method=hs.FIXED_COMB)
+    pip, _ = fs.msm(S, flags=hs.EXACT, method=hs.FIXED_PIPPENGER)
+    fs.close()
+    for k in range(K):
+        want = O.msm_naive(c, rows[k], bytes(pb), n)
+        assert comb[k] == want and pip[k] == want, k
+        assert GH.jac_to_bytes(c, jac[k]) == want
