@@ -298,22 +298,27 @@ template <> __device__ __forceinline__ int fixed_digit<FIXED_COMB16>(const uint3
     return (p & 0x8000u) ? (int)(p & 0x7fffu) + 1 : -(int)((~p & 0x7fffu) + 1);
 }
 
-// One wavefront per (MSM k, window w): S[k * windows + w] = sum_i T_i[d_kiw].
-template <class C, int WB>
+// One wavefront per (MSM k, CPW consecutive windows w): S[k * windows + w] = sum_i T_i[d_kiw].
+// 64 / CPW lanes share a window (lane l takes bases l, l + 64 / CPW, ...), then an LDS tree sum
+// per window; more windows per wavefront mean a shorter tree per addition.
+template <class C, int WB, int CPW>
 __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__restrict__ tables,
                                                           const uint32_t *__restrict__ scal, uint32_t n,
                                                           xyzz *__restrict__ S)
 {
     using F = typename C::F;
     constexpr int WINDOWS = FixedW<WB>::windows, ENTRIES = FixedW<WB>::entries;
+    constexpr uint32_t LANES = 64 / CPW, GROUPS = (WINDOWS + CPW - 1) / CPW;
+    static_assert(64 % CPW == 0, "windows per wavefront must divide 64");
     __shared__ xyzz sh[64];
-    const uint32_t k = blockIdx.x / WINDOWS;
-    const int w = (int)(blockIdx.x % WINDOWS);
+    const uint32_t k = blockIdx.x / GROUPS, lane = threadIdx.x % LANES;
+    const int w = (int)(blockIdx.x % GROUPS) * CPW + (int)(threadIdx.x / LANES);
+    const uint32_t nn = w < WINDOWS ? n : 0u; // lanes past the last window sum nothing
     const uint32_t *sk = scal + (size_t)k * n * 8;
     xyzz acc;
     bool acc_id = true;
     xyzz_identity<F>(acc);
-    for (uint32_t i = threadIdx.x; i < n; i += 64)
+    for (uint32_t i = lane; i < nn; i += LANES)
     {
         const int d = fixed_digit<WB>(sk + 8ull * i, w);
         if (d == 0)
@@ -343,9 +348,9 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
         xyzz_identity<F>(acc);
     store_xyzz(sh + threadIdx.x, acc);
     __syncthreads();
-    for (uint32_t h = 32; h > 0; h >>= 1)
+    for (uint32_t h = LANES / 2; h > 0; h >>= 1)
     {
-        if (threadIdx.x < h)
+        if (lane < h)
         {
             xyzz a, b;
             load_xyzz(a, sh + threadIdx.x);
@@ -355,11 +360,11 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0)
+    if (lane == 0 && w < WINDOWS)
     {
         xyzz r;
-        load_xyzz(r, sh);
-        store_xyzz(S + blockIdx.x, r);
+        load_xyzz(r, sh + threadIdx.x);
+        store_xyzz(S + (size_t)k * WINDOWS + w, r);
     }
 }
 
