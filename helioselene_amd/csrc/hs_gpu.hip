@@ -61,6 +61,23 @@ struct DevBuf
         cap = want;
         return 0;
     }
+    // exactly `bytes` (no growth slack), for buffers sized once such as fixed-base tables
+    int reserve_exact(size_t bytes)
+    {
+        if (bytes <= cap)
+            return 0;
+        if (p)
+            hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess)
+        {
+            p = nullptr;
+            return fail(HS_GPU_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+        cap = bytes;
+        return 0;
+    }
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
@@ -938,7 +955,7 @@ int fixed_alloc(hs_gpu_fixed **out, int curve, size_t n)
     auto *f = new hs_gpu_fixed();
     f->curve = curve;
     f->n = (uint32_t)n;
-    if ((r = f->tables.reserve(n * hs::FIXED_ENTRIES * 64)) || (r = f->points.reserve(n * sizeof(hs_gpu_jacobian))))
+    if ((r = f->points.reserve_exact(n * sizeof(hs_gpu_jacobian))))
     {
         hs_gpu_fixed_destroy(f);
         return r;
@@ -971,7 +988,7 @@ template <class C> int fixed_build_comb(hs_gpu_fixed &f, hipStream_t st)
 {
     constexpr int E = hs::FixedW<hs::FIXED_COMB16>::entries;
     int r;
-    if ((r = f.tables.reserve((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * 64)) ||
+    if ((r = f.tables.reserve_exact((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * 64)) ||
         (r = f.sums.reserve((size_t)f.n * 16 * 64)))
         return r;
     hipLaunchKernelGGL(hs::k_fixed_bases<C>, dim3((f.n + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
@@ -991,7 +1008,7 @@ template <class C, int WB> int fixed_build_tables_w(hs_gpu_fixed &f, uint64_t *d
     constexpr int E = hs::FixedW<WB>::entries;
     int r;
     // the affine bases are staged in the per-call scalar workspace
-    if ((r = f.tables.reserve((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * 64)))
+    if ((r = f.tables.reserve_exact((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * 64)))
         return r;
     hipLaunchKernelGGL(hs::k_fixed_bases<C>, dim3((f.n + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
                        f.scal.as<uint32_t>());
@@ -1181,6 +1198,8 @@ void slot_msm_fixed(int curve, hs_gpu_jacobian *result, const unsigned char *sca
         r = fail(HS_GPU_EINVAL, "null input with n > 0");
     if (!r)
         r = fixed_alloc(&f, curve, n);
+    if (!r)
+        r = f->tables.reserve_exact(n * hs::FIXED_ENTRIES * 64);
     if (!r)
     {
         // stage the caller's tables (n pointers to 16 reference affine entries) in one upload
