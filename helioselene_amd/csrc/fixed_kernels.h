@@ -273,28 +273,37 @@ __global__ void __launch_bounds__(256) k_fixed_prep(const uint32_t *__restrict__
     op[1] = make_uint4(k[4], k[5], k[6], k[7]);
 }
 
-// signed WB-bit digit of window w of a canonical scalar (< 2^255), in [-2^(WB-1), 2^(WB-1)]
-template <int WB> __device__ __forceinline__ int fixed_digit(const uint32_t *__restrict__ s, int w)
+// The signed WB-bit digit of window w of a canonical scalar (< 2^255), in [-2^(WB-1), 2^(WB-1)],
+// from scalar words wi = fixed_digit_word<WB>(w) (lo) and wi + 1 (hi, 0 past the top).
+template <int WB> __device__ __forceinline__ int fixed_digit_word(int w)
+{
+    return w == 0 ? 0 : (WB * w - 1) >> 5;
+}
+
+template <int WB> __device__ __forceinline__ int fixed_digit_from(uint32_t lo, uint32_t hi, int w)
 {
     constexpr uint32_t mask = (2u << WB) - 1; // the window's WB bits and the bit below it
     uint32_t v;
     if (w == 0)
-        v = (s[0] << 1) & mask;
+        v = (lo << 1) & mask;
     else
     {
-        const int bit = WB * w - 1, wi = bit >> 5, off = bit & 31;
-        const uint64_t lo = s[wi];
-        const uint64_t hi = wi + 1 < 8 ? s[wi + 1] : 0u;
-        v = (uint32_t)(((lo | (hi << 32)) >> off) & mask);
+        const int off = (WB * w - 1) & 31;
+        v = (uint32_t)((((uint64_t)lo | ((uint64_t)hi << 32)) >> off) & mask);
     }
     const int raw = (int)(v >> 1), cin = (int)(v & 1u);
     return raw + cin - ((raw >> (WB - 1)) << WB);
 }
 
-// Comb column r: +-(entry + 1) of the pattern, never 0.
-template <> __device__ __forceinline__ int fixed_digit<FIXED_COMB16>(const uint32_t *__restrict__ s, int r)
+// Comb column r (halfword r of the prepared row): +-(entry + 1) of the pattern, never 0.
+template <> __device__ __forceinline__ int fixed_digit_word<FIXED_COMB16>(int r)
 {
-    const uint32_t p = (s[r >> 1] >> (16 * (r & 1))) & 0xffffu;
+    return r >> 1;
+}
+
+template <> __device__ __forceinline__ int fixed_digit_from<FIXED_COMB16>(uint32_t lo, uint32_t, int r)
+{
+    const uint32_t p = (lo >> (16 * (r & 1))) & 0xffffu;
     return (p & 0x8000u) ? (int)(p & 0x7fffu) + 1 : -(int)((~p & 0x7fffu) + 1);
 }
 
@@ -318,13 +327,48 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
     xyzz acc;
     bool acc_id = true;
     xyzz_identity<F>(acc);
+    // software pipeline over this lane's bases: the scalar words two bases ahead and the table
+    // entry one base ahead are in flight during the current addition
+    const int wi = fixed_digit_word<WB>(w);
+    uint32_t lo = 0, hi = 0;
+    auto words = [&](uint32_t ii) {
+        const uint32_t *r = sk + 8ull * ii;
+        lo = r[wi];
+        hi = wi + 1 < 8 ? r[wi + 1] : 0u;
+    };
+    int dn = 0;
+    uint4 qn[4] = {};
+    auto entry = [&](uint32_t ii) {
+        dn = fixed_digit_from<WB>(lo, hi, w);
+        if (dn != 0)
+        {
+            const uint4 *p =
+                reinterpret_cast<const uint4 *>(tables + 16ull * (ii * ENTRIES + (uint32_t)((dn < 0 ? -dn : dn) - 1)));
+            qn[0] = p[0];
+            qn[1] = p[1];
+            qn[2] = p[2];
+            qn[3] = p[3];
+        }
+    };
+    if (lane < nn)
+    {
+        words(lane);
+        entry(lane);
+    }
+    if (lane + LANES < nn)
+        words(lane + LANES);
     for (uint32_t i = lane; i < nn; i += LANES)
     {
-        const int d = fixed_digit<WB>(sk + 8ull * i, w);
+        const int d = dn;
+        const uint4 q0 = qn[0], q1 = qn[1], q2 = qn[2], q3 = qn[3];
+        if (i + LANES < nn)
+        {
+            entry(i + LANES);
+            if (i + 2 * LANES < nn)
+                words(i + 2 * LANES);
+        }
         if (d == 0)
             continue;
-        const uint4 *p = reinterpret_cast<const uint4 *>(tables + 16ull * (i * ENTRIES + (uint32_t)((d < 0 ? -d : d) - 1)));
-        const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
         const uint32_t xw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
         const uint32_t yw[8] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
         uint32_t any = 0;
