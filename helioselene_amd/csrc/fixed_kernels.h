@@ -430,7 +430,10 @@ __global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S,
     xyzz_to_jac_q<F>(acc, s);
     for (int w = WINDOWS - 2; w >= 0; w--)
     {
-        jac_dbl_n_q<F>(acc, FixedW<WB>::step);
+        if constexpr (FixedW<WB>::step == 1)
+            jac_dbl_q<F>(acc); // 4 product levels; the pipelined form needs 5 for one doubling
+        else
+            jac_dbl_n_q<F>(acc, FixedW<WB>::step);
         load_xyzz(s, sk + w);
         xyzz_to_jac_q<F>(t, s);
         jac_add_q<F>(acc, acc, t);
