@@ -403,7 +403,8 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     uint16_t *digits = ws.digits.as<uint16_t>();
     uint32_t *offsets = ws.offsets.as<uint32_t>(), *sorted = ws.sorted.as<uint32_t>(), *entries = ws.entries.as<uint32_t>(),
              *cnt_a = ws.cnt_a.as<uint32_t>(), *off_a = ws.off_a.as<uint32_t>();
-    hs::xyzz *buckets = ws.buckets.as<hs::xyzz>(), *seg = ws.seg.as<hs::xyzz>(), *win = ws.win.as<hs::xyzz>();
+    hs::xyzz *buckets = ws.buckets.as<hs::xyzz>(), *seg = ws.seg.as<hs::xyzz>();
+    hs::jac *win = ws.win.as<hs::jac>(); // window sums, Jacobian
     uint32_t *big_list = ws.big_list.as<uint32_t>(), *big_count = ws.big_count.as<uint32_t>();
 
     const dim3 blk(256), grd((nn + 255) / 256);
@@ -537,7 +538,8 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
     const uint32_t N = g.N;
     hs::BatchDesc *d = g.desc.as<hs::BatchDesc>();
     uint32_t *aff = g.aff.as<uint32_t>(), *sorted = g.sorted.as<uint32_t>(), *offsets = g.offsets.as<uint32_t>();
-    hs::xyzz *buckets = g.buckets.as<hs::xyzz>(), *seg = g.seg.as<hs::xyzz>(), *win = g.win.as<hs::xyzz>();
+    hs::xyzz *buckets = g.buckets.as<hs::xyzz>(), *seg = g.seg.as<hs::xyzz>();
+    hs::jac *win = g.win.as<hs::jac>(); // window sums, Jacobian
     hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, g.long_count.as<uint32_t>(), 1u, 0u);
     hipLaunchKernelGGL(hs::k_ingest_batch<C>, dim3((N + 255) / 256), blk, 0, st, d, g.M, N, aff, g.isid.as<uint8_t>());
     hipLaunchKernelGGL(hs::k_recode_batch<C>, dim3((N + 255) / 256), blk, 0, st, d, g.M, N, g.isid.as<uint8_t>(), g.c, g.W,
@@ -2075,17 +2077,28 @@ int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t 
         return 0;
     }
     DevBuf &src = which == 0 ? ws.win : which == 1 ? ws.seg : ws.buckets;
-    size_t n = std::min(cap_bytes / 32, src.cap / sizeof(hs::xyzz));
+    // window sums are Jacobian, segment and bucket sums xyzz
+    size_t n = std::min(cap_bytes / 32, src.cap / (which == 0 ? sizeof(hs::jac) : sizeof(hs::xyzz)));
     int r;
     if ((r = cx.probe.reserve(n * 32 + 64)))
         return r;
     const uint32_t nn = (uint32_t)n;
-    if (curve == HS_GPU_HELIOS)
-        hipLaunchKernelGGL(hs::k_encode_xyzz<hs::HeliosC>, dim3((nn + 63) / 64), dim3(64), 0, cx.stream,
-                           src.as<hs::xyzz>(), nn, cx.probe.as<uint32_t>());
+    const dim3 grid((nn + 63) / 64);
+    if (which == 0)
+    {
+        if (curve == HS_GPU_HELIOS)
+            hipLaunchKernelGGL(hs::k_encode_jac<hs::HeliosC>, grid, dim3(64), 0, cx.stream, src.as<hs::jac>(), nn,
+                               cx.probe.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(hs::k_encode_jac<hs::SeleneC>, grid, dim3(64), 0, cx.stream, src.as<hs::jac>(), nn,
+                               cx.probe.as<uint32_t>());
+    }
+    else if (curve == HS_GPU_HELIOS)
+        hipLaunchKernelGGL(hs::k_encode_xyzz<hs::HeliosC>, grid, dim3(64), 0, cx.stream, src.as<hs::xyzz>(), nn,
+                           cx.probe.as<uint32_t>());
     else
-        hipLaunchKernelGGL(hs::k_encode_xyzz<hs::SeleneC>, dim3((nn + 63) / 64), dim3(64), 0, cx.stream,
-                           src.as<hs::xyzz>(), nn, cx.probe.as<uint32_t>());
+        hipLaunchKernelGGL(hs::k_encode_xyzz<hs::SeleneC>, grid, dim3(64), 0, cx.stream, src.as<hs::xyzz>(), nn,
+                           cx.probe.as<uint32_t>());
     HS_CHECK(hipGetLastError());
     HS_CHECK(hipStreamSynchronize(cx.stream));
     HS_CHECK(hipMemcpy(out, cx.probe.p, n * 32, hipMemcpyDeviceToHost));

@@ -634,7 +634,7 @@ __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buc
 // of its threads summing identities.)
 template <class C>
 __global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win,
-                                                            uint32_t w_lo, uint32_t nwin, xyzz *__restrict__ win_out)
+                                                            uint32_t w_lo, uint32_t nwin, jac *__restrict__ win_out)
 {
     using F = typename C::F;
     __shared__ xyzz sh[BIG_THREADS];
@@ -668,8 +668,12 @@ __global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restri
     }
     if (lt == 0 && live)
     {
+        // the window sum leaves in Jacobian form, converted here in parallel over windows rather
+        // than inside the serial Horner chain
         load_xyzz(acc, sh + threadIdx.x);
-        store_xyzz(win_out + w, acc);
+        jac J;
+        xyzz_to_jac<F>(J, acc);
+        win_out[w] = J;
     }
 }
 
@@ -705,19 +709,17 @@ __device__ __forceinline__ void force_vgpr(xyzz &p)
 // from the identity, otherwise from the Jacobian value in *total; the result goes back to *total.
 // One quad of lanes runs the chain (quad-parallel doublings and additions).
 template <class C>
-__global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out, int w_lo, int w_hi, int c, int first,
+__global__ void __launch_bounds__(64) k_horner(const jac *__restrict__ win_out, int w_lo, int w_hi, int c, int first,
                                                jac *total)
 {
     using F = typename C::F;
     if (threadIdx.x >= 4 || blockIdx.x != 0)
         return;
-    xyzz s;
     jac acc, t;
     int w = w_hi - 1;
     if (first)
     {
-        load_xyzz(s, win_out + w);
-        xyzz_to_jac_q<F>(acc, s);
+        acc = win_out[w];
         w--;
     }
     else
@@ -725,8 +727,7 @@ __global__ void __launch_bounds__(64) k_horner(const xyzz *__restrict__ win_out,
     for (; w >= w_lo; w--)
     {
         jac_dbl_n_q<F>(acc, c);
-        load_xyzz(s, win_out + w);
-        xyzz_to_jac_q<F>(t, s);
+        t = win_out[w];
         jac_add_q<F>(acc, acc, t);
     }
     if (threadIdx.x == 0)
@@ -897,6 +898,22 @@ __global__ void __launch_bounds__(64) k_encode_xyzz(const xyzz *__restrict__ p, 
     load_xyzz(q, p + i);
     aff a;
     bool pt = xyzz_to_aff<F>(a, q);
+    uint32_t e[8];
+    encode_aff<F>(e, a, pt);
+    for (int k = 0; k < 8; k++)
+        out32[8ull * i + k] = e[k];
+}
+// Jacobian points -> 32-byte encodings (identity -> 0^32); for debugging dumps.
+template <class C>
+__global__ void __launch_bounds__(64) k_encode_jac(const jac *__restrict__ p, uint32_t n, uint32_t *__restrict__ out32)
+{
+    using F = typename C::F;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const jac q = p[i];
+    aff a;
+    bool pt = jac_to_aff<F>(a, q);
     uint32_t e[8];
     encode_aff<F>(e, a, pt);
     for (int k = 0; k < 8; k++)
@@ -1087,23 +1104,20 @@ __global__ void __launch_bounds__(256) k_window_sort(const BatchDesc *__restrict
 
 // Horner per MSM (one quad of lanes each) over its W window sums.
 template <class C>
-__global__ void __launch_bounds__(64) k_horner_batch(const xyzz *__restrict__ win_out, uint32_t M, int W, int c,
+__global__ void __launch_bounds__(64) k_horner_batch(const jac *__restrict__ win_out, uint32_t M, int W, int c,
                                                      jac *__restrict__ total)
 {
     using F = typename C::F;
     const uint32_t m = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (m >= M)
         return;
-    const xyzz *ws = win_out + (size_t)m * W;
-    xyzz s;
+    const jac *ws = win_out + (size_t)m * W;
     jac acc, t;
-    load_xyzz(s, ws + (W - 1));
-    xyzz_to_jac_q<F>(acc, s);
+    acc = ws[W - 1];
     for (int w = W - 2; w >= 0; w--)
     {
         jac_dbl_n_q<F>(acc, c);
-        load_xyzz(s, ws + w);
-        xyzz_to_jac_q<F>(t, s);
+        t = ws[w];
         jac_add_q<F>(acc, acc, t);
     }
     if ((threadIdx.x & 3) == 0)
