@@ -307,13 +307,14 @@ template <> __device__ __forceinline__ int fixed_digit_from<FIXED_COMB16>(uint32
     return (p & 0x8000u) ? (int)(p & 0x7fffu) + 1 : -(int)((~p & 0x7fffu) + 1);
 }
 
-// One wavefront per (MSM k, CPW consecutive windows w): S[k * windows + w] = sum_i T_i[d_kiw].
+// One wavefront per (MSM k, CPW consecutive windows w): S[k * windows + w] = sum_i T_i[d_kiw]
+// (Jacobian).
 // 64 / CPW lanes share a window (lane l takes bases l, l + 64 / CPW, ...), then an LDS tree sum
 // per window; more windows per wavefront mean a shorter tree per addition.
 template <class C, int WB, int CPW>
 __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__restrict__ tables,
                                                           const uint32_t *__restrict__ scal, uint32_t n,
-                                                          xyzz *__restrict__ S)
+                                                          jac *__restrict__ S)
 {
     using F = typename C::F;
     constexpr int WINDOWS = FixedW<WB>::windows, ENTRIES = FixedW<WB>::entries;
@@ -406,16 +407,19 @@ __global__ void __launch_bounds__(64) k_fixed_window_sums(const uint32_t *__rest
     }
     if (lane == 0 && w < WINDOWS)
     {
+        // stored in Jacobian form, so the serial Horner chain needs no conversion
         xyzz r;
         load_xyzz(r, sh + threadIdx.x);
-        store_xyzz(S + (size_t)k * WINDOWS + w, r);
+        jac J;
+        xyzz_to_jac<F>(J, r);
+        S[(size_t)k * WINDOWS + w] = J;
     }
 }
 
 // One quad of lanes per MSM: Horner over the window sums (step shared doublings per window),
 // then the 32-byte encoding and the reference struct (affine, Z = 1; identity 1:1:0).
 template <class C, int WB>
-__global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S, uint32_t K,
+__global__ void __launch_bounds__(64) k_fixed_horner(const jac *__restrict__ S, uint32_t K,
                                                      uint32_t *__restrict__ out32, uint64_t *__restrict__ outjac)
 {
     using F = typename C::F;
@@ -423,19 +427,16 @@ __global__ void __launch_bounds__(64) k_fixed_horner(const xyzz *__restrict__ S,
     const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     if (k >= K)
         return;
-    const xyzz *sk = S + (size_t)k * WINDOWS;
-    xyzz s;
+    const jac *sk = S + (size_t)k * WINDOWS;
     jac acc, t;
-    load_xyzz(s, sk + WINDOWS - 1);
-    xyzz_to_jac_q<F>(acc, s);
+    acc = sk[WINDOWS - 1];
     for (int w = WINDOWS - 2; w >= 0; w--)
     {
         if constexpr (FixedW<WB>::step == 1)
             jac_dbl_q<F>(acc); // 4 product levels; the pipelined form needs 5 for one doubling
         else
             jac_dbl_n_q<F>(acc, FixedW<WB>::step);
-        load_xyzz(s, sk + w);
-        xyzz_to_jac_q<F>(t, s);
+        t = sk[w];
         jac_add_q<F>(acc, acc, t);
     }
     if ((threadIdx.x & 3) != 0)

@@ -1057,16 +1057,16 @@ int fixed_comb_enqueue(hs_gpu_fixed &f, const void *d_scalars, size_t count, uns
     constexpr int W = hs::FixedW<WB>::windows;
     int r;
     const size_t rows = count * f.n;
-    if ((r = f.scal.reserve(rows * 32)) || (r = f.sums.reserve(count * W * sizeof(hs::xyzz))))
+    if ((r = f.scal.reserve(rows * 32)) || (r = f.sums.reserve(count * W * sizeof(hs::jac))))
         return r;
     const uint32_t compat = (!(flags & HS_GPU_EXACT) && ref_window(f.n) == 8) ? 1u : 0u;
     hipLaunchKernelGGL(hs::k_fixed_prep<C>, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st,
                        (const uint32_t *)d_scalars, rows, f.scal.as<uint32_t>(), compat,
                        WB == hs::FIXED_COMB16 ? 1u : 0u);
     hipLaunchKernelGGL((hs::k_fixed_window_sums<C, WB, FIXED_CPW>), dim3((unsigned)(count * ((W + FIXED_CPW - 1) / FIXED_CPW))), dim3(64), 0, st,
-                       f.tables.as<uint32_t>(), f.scal.as<uint32_t>(), f.n, f.sums.as<hs::xyzz>());
+                       f.tables.as<uint32_t>(), f.scal.as<uint32_t>(), f.n, f.sums.as<hs::jac>());
     hipLaunchKernelGGL((hs::k_fixed_horner<C, WB>), dim3((unsigned)((4 * count + 63) / 64)), dim3(64), 0, st,
-                       f.sums.as<hs::xyzz>(), (uint32_t)count, (uint32_t *)d_out32, (uint64_t *)d_outjac);
+                       f.sums.as<hs::jac>(), (uint32_t)count, (uint32_t *)d_out32, (uint64_t *)d_outjac);
     HS_CHECK(hipGetLastError());
     return 0;
 }
