@@ -206,6 +206,23 @@ namespace
             O::map(&out[i], u + 32 * i);
     }
 
+    // The FCMP++ benchmark's bases: G, 2G, 4G, ... by successive doublings (src/benchmark_fcmpp.cpp:51-69)
+    template <class O> void dbl_chain(typename O::jac *out, size_t n)
+    {
+        if (n == 0)
+            return;
+        O::generator(&out[0]);
+        for (size_t i = 1; i < n; i++)
+            O::dbl(&out[i], &out[i - 1]);
+    }
+
+    template <class O> void msm_jac_bytes(unsigned char *o, const unsigned char *s, const typename O::jac *p, size_t n)
+    {
+        typename O::jac r;
+        O::msm(&r, s, p, n);
+        O::tobytes(o, &r);
+    }
+
     template <class O> int to_jac(typename O::jac *out, const unsigned char *pts, size_t n)
     {
         std::vector<typename O::jac> p;
@@ -269,6 +286,17 @@ extern "C"
     void hsref_selene_map_to_curve_bytes(unsigned char *o, const unsigned char *u, size_t n) { map_bytes<SeleneOps>(o, u, n); }
     void hsref_helios_map_to_curve_jac(helios_jacobian *o, const unsigned char *u, size_t n) { map_jac<HeliosOps>(o, u, n); }
     void hsref_selene_map_to_curve_jac(selene_jacobian *o, const unsigned char *u, size_t n) { map_jac<SeleneOps>(o, u, n); }
+
+    void hsref_helios_dbl_chain(helios_jacobian *o, size_t n) { dbl_chain<HeliosOps>(o, n); }
+    void hsref_selene_dbl_chain(selene_jacobian *o, size_t n) { dbl_chain<SeleneOps>(o, n); }
+    void hsref_helios_msm_jac_bytes(unsigned char *o, const unsigned char *s, const helios_jacobian *p, size_t n)
+    {
+        msm_jac_bytes<HeliosOps>(o, s, p, n);
+    }
+    void hsref_selene_msm_jac_bytes(unsigned char *o, const unsigned char *s, const selene_jacobian *p, size_t n)
+    {
+        msm_jac_bytes<SeleneOps>(o, s, p, n);
+    }
 
     void hsref_helios_reduce_wide(unsigned char *o, const unsigned char *w) { HeliosOps::reduce_wide(o, w); }
     void hsref_selene_reduce_wide(unsigned char *o, const unsigned char *w) { SeleneOps::reduce_wide(o, w); }

@@ -64,6 +64,9 @@ def ref():
             getattr(_r, f"hsref_{c}_time_msm").argtypes = [ctypes.c_char_p, ctypes.c_char_p, sz,
                                                            ctypes.c_int, ctypes.c_int]
             getattr(_r, f"hsref_{c}_time_msm").restype = ctypes.c_double
+            getattr(_r, f"hsref_{c}_dbl_chain").argtypes = [ctypes.c_void_p, sz]
+            getattr(_r, f"hsref_{c}_msm_jac_bytes").argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, sz]
+            getattr(_r, f"hsref_{c}_map_to_curve_jac").argtypes = [ctypes.c_void_p, ctypes.c_char_p, sz]
     return _r
 
 
@@ -196,6 +199,45 @@ def gen_kG_points(c, seed, n):
     for i in range(n):
         out += scalar_mul(c, ks[32 * i:32 * i + 32], g)
     return bytes(out)
+
+
+def ref_dbl_chain(c, n):
+    """The reference's own FCMP++ bases G, 2G, 4G, ... (its Jacobian structs, Z != 1), uint64 (n, 15)."""
+    import numpy as np
+
+    out = np.zeros((n, 15), dtype=np.uint64)
+    getattr(ref(), f"hsref_{CURVES[c]}_dbl_chain")(out.ctypes.data, n)
+    return out
+
+
+def ref_map_to_curve_jac(c, u):
+    """The reference's SSWU map of n 32-byte inputs as its Jacobian structs, uint64 (n, 15)."""
+    import numpy as np
+
+    n = len(u) // 32
+    out = np.zeros((n, 15), dtype=np.uint64)
+    getattr(ref(), f"hsref_{CURVES[c]}_map_to_curve_jac")(out.ctypes.data, bytes(u), n)
+    return out
+
+
+def ref_map_to_curve_bytes(c, u):
+    """The reference's SSWU map of n 32-byte inputs, encoded (helios_tobytes)."""
+    n = len(u) // 32
+    out = _buf(32 * n)
+    getattr(ref(), f"hsref_{CURVES[c]}_map_to_curve_bytes")(out, bytes(u), n)
+    return out.raw
+
+
+def ref_msm_jac(c, scalars, pts):
+    """Reference MSM bytes over the reference's Jacobian structs (uint64 (n, 15)) and scalars
+    (uint8 (n, 32))."""
+    import numpy as np
+
+    s = np.ascontiguousarray(scalars, dtype=np.uint8)
+    p = np.ascontiguousarray(pts, dtype=np.uint64)
+    out = _buf(32)
+    getattr(ref(), f"hsref_{CURVES[c]}_msm_jac_bytes")(out, s.ctypes.data, p.ctypes.data, p.shape[0])
+    return out.raw
 
 
 def ref_msm(c, scalars, points, n):
