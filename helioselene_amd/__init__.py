@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libhs_gpu.so")
 HELIOS, SELENE = 0, 1
 CURVES = {"helios": HELIOS, "selene": SELENE, HELIOS: HELIOS, SELENE: SELENE}
 EXACT = 1
+ERASE = 2  # zero the device copies of scalars and digits after use
 
 # group orders and coordinate moduli (reference helios_constants.h:57-64, selene_constants.h:56-64)
 P = (1 << 255) - 19
@@ -27,6 +28,11 @@ ORDER = {HELIOS: Q, SELENE: P}
 FIELD = {HELIOS: P, SELENE: Q}
 
 _lib = None
+
+# hs_gpu_cpu_msm_fn / hs_gpu_cpu_msm_fixed_fn: (result, scalars, points or tables, n)
+CPU_MSM_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+CPU_MSM_FIXED_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+_fallbacks = {}
 
 
 class HsGpuError(RuntimeError):
@@ -96,6 +102,10 @@ def lib():
         L.hs_gpu_tune_get.argtypes = [ctypes.c_int, ctypes.c_int, sz, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.hs_gpu_debug_dump.argtypes = [ctypes.c_int, ctypes.c_int, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
+        L.hs_gpu_set_cpu_fallback.argtypes = [ctypes.c_int, CPU_MSM_FN, CPU_MSM_FIXED_FN]
+        L.hs_gpu_fallback_count.restype = ctypes.c_uint64
+        L.hs_gpu_set_device_limit.argtypes = [sz]
+        L.hs_gpu_device_bytes.restype = sz
         _lib = L
     return _lib
 
@@ -199,6 +209,35 @@ def msm_slot(curve, scalars, points):
 def msm_device(curve, d_scalars, d_points, n, d_out32, d_out_jac=None, flags=0, stream=None):
     """Device-resident MSM: arguments are device addresses (ints, e.g. torch tensor data_ptr())."""
     _check(lib().hs_gpu_msm_device(CURVES[curve], d_scalars, d_points, n, flags, d_out32, d_out_jac, stream))
+
+
+def set_cpu_fallback(curve, msm=None, msm_fixed=None):
+    """Registers CPU functions the void drop-ins call when the device fails (hs_gpu_set_cpu_fallback).
+    msm(result_addr, scalars_addr, points_addr, n) and msm_fixed(result_addr, scalars_addr,
+    tables_addr, n) receive raw addresses; None clears.  The callbacks are kept alive here."""
+    c = CURVES[curve]
+    f1 = CPU_MSM_FN(msm) if msm else CPU_MSM_FN()
+    f2 = CPU_MSM_FIXED_FN(msm_fixed) if msm_fixed else CPU_MSM_FIXED_FN()
+    _fallbacks[c] = (f1, f2)
+    _check(lib().hs_gpu_set_cpu_fallback(c, f1, f2))
+
+
+def fallback_count():
+    return lib().hs_gpu_fallback_count()
+
+
+def set_device_limit(nbytes):
+    """Cap on the device bytes the library may hold (0 = none)."""
+    _check(lib().hs_gpu_set_device_limit(nbytes))
+
+
+def device_bytes():
+    return lib().hs_gpu_device_bytes()
+
+
+def trim():
+    """Frees the library's cached workspaces (after waiting for its streams)."""
+    _check(lib().hs_gpu_trim())
 
 
 def synchronize():

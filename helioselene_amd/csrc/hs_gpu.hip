@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <chrono>
@@ -39,25 +40,62 @@ int fail(int code, const std::string &msg)
             return fail(HS_GPU_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                   \
     } while (0)
 
-// A growable device buffer.
+// Device bytes the library holds, and an optional cap on them (hs_gpu_set_device_limit, or the
+// HS_GPU_DEVICE_LIMIT environment variable in bytes).  An allocation that would pass the cap
+// fails with HS_GPU_ENOMEM, as a refused hipMalloc does.
+size_t env_size(const char *name)
+{
+    const char *v = std::getenv(name);
+    return v ? (size_t)std::strtoull(v, nullptr, 10) : 0;
+}
+std::atomic<size_t> g_dev_held{0};
+std::atomic<size_t> g_dev_limit{env_size("HS_GPU_DEVICE_LIMIT")};
+
+int dev_malloc(void **p, size_t bytes)
+{
+    const size_t lim = g_dev_limit.load();
+    if (lim && g_dev_held.load() + bytes > lim)
+    {
+        *p = nullptr;
+        return fail(HS_GPU_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes would pass the library's " +
+                                       std::to_string(lim) + "-byte limit");
+    }
+    if (hipMalloc(p, bytes) != hipSuccess)
+    {
+        *p = nullptr;
+        return fail(HS_GPU_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    g_dev_held += bytes;
+    return 0;
+}
+
+void dev_free(void *p, size_t bytes)
+{
+    if (!p)
+        return;
+    hipFree(p);
+    g_dev_held -= bytes;
+}
+
+// A growable device buffer (freed only by release()).
 struct DevBuf
 {
     void *p = nullptr;
     size_t cap = 0;
+    void release()
+    {
+        dev_free(p, cap);
+        p = nullptr;
+        cap = 0;
+    }
     int reserve(size_t bytes)
     {
         if (bytes <= cap)
             return 0;
-        if (p)
-            hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = bytes + bytes / 4 + 4096;
-        if (hipMalloc(&p, want) != hipSuccess)
-        {
-            p = nullptr;
-            return fail(HS_GPU_ENOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
-        }
+        release();
+        const size_t want = bytes + bytes / 4 + 4096;
+        if (int r = dev_malloc(&p, want))
+            return r;
         cap = want;
         return 0;
     }
@@ -66,20 +104,36 @@ struct DevBuf
     {
         if (bytes <= cap)
             return 0;
-        if (p)
-            hipFree(p);
-        p = nullptr;
-        cap = 0;
-        if (hipMalloc(&p, bytes) != hipSuccess)
-        {
-            p = nullptr;
-            return fail(HS_GPU_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
-        }
+        release();
+        if (int r = dev_malloc(&p, bytes))
+            return r;
         cap = bytes;
         return 0;
     }
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
+
+// Overwrites host memory holding secrets; the volatile stores cannot be elided.
+void secure_erase(void *p, size_t bytes)
+{
+    volatile unsigned char *q = (volatile unsigned char *)p;
+    for (size_t i = 0; i < bytes; i++)
+        q[i] = 0;
+}
+
+// Stream-ordered zeroing of `bytes` (rounded up to whole words) with the fill kernel, which
+// stream capture records as an ordinary kernel node.
+void zero_dev(void *p, size_t bytes, hipStream_t st)
+{
+    const size_t words = (bytes + 3) / 4;
+    if (!p || !words)
+        return;
+    for (size_t off = 0; off < words; off += (size_t)1 << 30)
+    {
+        const uint32_t k = (uint32_t)std::min<size_t>(words - off, (size_t)1 << 30);
+        hipLaunchKernelGGL(hs::k_fill_u32, dim3((k + 255) / 256), dim3(256), 0, st, (uint32_t *)p + off, k, 0u);
+    }
+}
 
 struct Plan
 {
@@ -240,6 +294,13 @@ struct Workspace
     static constexpr int MAXQ = 8;
     hipEvent_t ev_group[MAXQ] = {}, ev_done = nullptr;
 
+    void release_all()
+    {
+        for (DevBuf *d : {&aff, &isid, &digits, &counts, &offsets, &block_sums, &sorted, &buckets, &seg, &win, &big_list,
+                          &big_count, &entries, &cnt_a, &off_a, &total, &chunk_bucket, &bnd, &parts})
+            d->release();
+    }
+
     int ensure_aux()
     {
         if (aux)
@@ -252,12 +313,21 @@ struct Workspace
     }
 };
 
+// Staging of one host-pointer MSM in flight: a stream of its own (and with it a workspace of its
+// own) plus the upload and result buffers.  Concurrent host callers each take one from a pool.
+struct HostStage
+{
+    hipStream_t st = nullptr;
+    DevBuf scalars, points, out;
+};
+
 struct Ctx
 {
     int device = -1;
     hipStream_t stream = nullptr;
     std::mutex mu;
     DevBuf scalars, points, out32, outjac, status, nbad, probe;
+    std::vector<HostStage *> stages_free, stages_all;
     std::vector<std::pair<hipStream_t, Workspace *>> wss;
     bool ready = false;
 
@@ -333,6 +403,26 @@ Ctx &ctx()
 {
     static Ctx c;
     return c;
+}
+
+// A free staging slot (cx.mu held); nullptr with the error set when none can be made.
+HostStage *take_stage(Ctx &cx)
+{
+    if (!cx.stages_free.empty())
+    {
+        HostStage *h = cx.stages_free.back();
+        cx.stages_free.pop_back();
+        return h;
+    }
+    auto *h = new HostStage();
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
+    {
+        delete h;
+        fail(HS_GPU_EDEVICE, "hipStreamCreate failed");
+        return nullptr;
+    }
+    cx.stages_all.push_back(h);
+    return h;
 }
 
 int scan_exclusive(Workspace &cx, const uint32_t *in, uint32_t *out, uint32_t len, hipStream_t st)
@@ -467,6 +557,13 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         hipLaunchKernelGGL(hs::k_horner<C>, dim3(1), dim3(64), 0, cst, win, w_lo, w_hi, P.c, q == 0 ? 1 : 0, total);
     }
     hipLaunchKernelGGL(hs::k_finalize<C>, dim3(1), dim3(64), 0, cst, total, d_out32, d_outjac);
+    if (flags & HS_GPU_ERASE)
+    {
+        // the digits and the sorted entries (point index and digit sign) are functions of the scalars
+        zero_dev(digits, n * 2 * (size_t)P.W, cst);
+        zero_dev(entries, n * 4 * (size_t)P.W, cst);
+        zero_dev(sorted, n * 4 * (size_t)P.W, cst);
+    }
     if (Q > 1)
     {
         HS_CHECK(hipEventRecord(ws.ev_done, cst));
@@ -593,6 +690,10 @@ int batch_enqueue(hs_gpu_batch &b, hipStream_t st)
     return 0;
 }
 
+// Entry slots, bucket ids and point indices of one batch group stay below this (bit 31 of an entry
+// is the digit sign, 0xffffffff marks padding).
+constexpr size_t BATCH_GROUP_LIMIT = (size_t)1 << 31;
+
 // Plans a batch.  With `arena`, every device buffer is carved from it and descriptors are
 // uploaded asynchronously on `upload` (their host copies live in the plan); with `workers`
 // (and `events`, one more than workers) the plan runs on the caller's streams.
@@ -644,6 +745,9 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
     }
     std::vector<std::pair<int, int>> keys;
     std::vector<hs::BatchDesc> empties;
+    // HS_GPU_BATCH_GROUP_LIMIT lowers the group limit (tests of the split at small sizes)
+    const size_t lim_env = env_size("HS_GPU_BATCH_GROUP_LIMIT");
+    const size_t group_limit = lim_env && lim_env < BATCH_GROUP_LIMIT ? lim_env : BATCH_GROUP_LIMIT;
     for (size_t i = 0; i < count; i++)
     {
         const hs_gpu_msm_desc &d = descs[i];
@@ -668,23 +772,37 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
         TuneEntry te;
         const bool tuned = tune_lookup(1, d.curve, d.n, te);
         const int c = cmap[d.curve][want_c[i]];
+        const size_t Wc = (256 + c - 1) / c, Bc = (size_t)1 << (c - 1);
+        // A group's entry slots (N * W), bucket ids (M * W * B) and point indices are 32-bit values
+        // with bit 31 kept for the digit sign, so a group holds fewer than 2^31 of each: a member
+        // that would pass that limit starts another group of the same (curve, width).
         BatchGroup *g = nullptr;
-        for (size_t k = 0; k < keys.size(); k++)
+        for (size_t k = keys.size(); k-- > 0;)
             if (keys[k].first == d.curve && keys[k].second == c)
-                g = b->groups[k];
+            {
+                BatchGroup *h = b->groups[k];
+                if (((size_t)h->N + d.n) * Wc < group_limit && ((size_t)h->hdesc.size() + 1) * Wc * Bc < group_limit)
+                    g = h;
+                break;
+            }
         if (!g)
         {
             keys.emplace_back(d.curve, c);
             g = new BatchGroup();
             g->curve = d.curve;
             g->c = c;
-            g->W = (256 + c - 1) / c;
-            g->B = 1u << (c - 1);
-            const uint32_t G = tuned ? (uint32_t)te.G : 8u;
-            g->G = g->B < G ? g->B : G;
-            g->L = tuned && te.L > 0 ? (uint32_t)te.L : 32u;
+            g->W = (int)Wc;
+            g->B = (uint32_t)Bc;
+            g->G = 0;
+            g->L = 0;
             b->groups.push_back(g);
         }
+        // the group runs with the largest run length and segment size of its members: its buffers
+        // (chunk bounds, segment sums) are then no larger than the members' footprints together,
+        // which is what the serving loop admits by
+        const uint32_t Gi = tuned ? (uint32_t)te.G : 8u, Li = tuned && te.L > 0 ? (uint32_t)te.L : 32u;
+        g->G = std::max(g->G, Gi);
+        g->L = std::max(g->L, Li);
         hs::BatchDesc bd;
         bd.scalars = (const uint32_t *)d.scalars;
         bd.points = (const uint64_t *)d.points;
@@ -701,6 +819,8 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
     }
     for (BatchGroup *g : b->groups)
     {
+        g->G = std::max(1u, std::min(g->B, g->G));
+        g->L = std::max(1u, g->L);
         g->M = (uint32_t)g->hdesc.size();
         g->V = g->M * (uint32_t)g->W;
         g->nb = g->V * g->B;
@@ -880,41 +1000,94 @@ int msm_host(int curve, hs_gpu_jacobian *result, unsigned char *out32, const uns
     }
     if (!scalars || !points)
         return fail(HS_GPU_EINVAL, "null input with n > 0");
+    if (n > SIZE_MAX / sizeof(hs_gpu_jacobian))
+        return fail(HS_GPU_EINVAL, "n too large");
     Ctx &cx = ctx();
-    std::lock_guard<std::mutex> lk(cx.mu);
-    int r;
-    if ((r = cx.init()))
+    HostStage *hsg = nullptr;
+    int r = 0;
+    {
+        // launch under the lock (workspace lookup, profiling state); wait outside it, so concurrent
+        // callers' MSMs overlap on the device
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if ((r = cx.init()))
+            return r;
+        if (!(hsg = take_stage(cx)))
+            return HS_GPU_EDEVICE;
+        const size_t ob = 32 + sizeof(hs_gpu_jacobian);
+        hipStream_t st = hsg->st;
+        if (!(r = hsg->scalars.reserve(n * 32)) && !(r = hsg->points.reserve(n * sizeof(hs_gpu_jacobian))) &&
+            !(r = hsg->out.reserve(ob)))
+        {
+            if (hipMemcpyAsync(hsg->scalars.p, scalars, n * 32, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(hsg->points.p, points, n * sizeof(hs_gpu_jacobian), hipMemcpyHostToDevice, st) != hipSuccess)
+                r = fail(HS_GPU_EDEVICE, "input upload failed");
+            unsigned char *o = hsg->out.as<unsigned char>();
+            if (!r)
+                r = msm_device_any(curve, cx, hsg->scalars.p, hsg->points.p, n, flags, o, o + 32, st);
+            if (flags & HS_GPU_ERASE)
+                zero_dev(hsg->scalars.p, n * 32, st);
+        }
+        if (r)
+        {
+            hipStreamSynchronize(hsg->st);
+            cx.stages_free.push_back(hsg);
+            return r;
+        }
+    }
+    unsigned char o[32 + sizeof(hs_gpu_jacobian)];
+    if (hipMemcpyAsync(o, hsg->out.p, sizeof o, hipMemcpyDeviceToHost, hsg->st) != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, "result copy failed");
+    const hipError_t e = hipStreamSynchronize(hsg->st);
+    if (!r && e != hipSuccess)
+        r = fail(HS_GPU_EDEVICE, std::string("MSM failed: ") + hipGetErrorString(e));
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        cx.stages_free.push_back(hsg);
+    }
+    if (r)
         return r;
-    if ((r = cx.scalars.reserve(n * 32)) || (r = cx.points.reserve(n * sizeof(hs_gpu_jacobian))) ||
-        (r = cx.out32.reserve(64)) || (r = cx.outjac.reserve(sizeof(hs_gpu_jacobian))))
-        return r;
-    hipStream_t st = cx.stream;
-    HS_CHECK(hipMemcpyAsync(cx.scalars.p, scalars, n * 32, hipMemcpyHostToDevice, st));
-    HS_CHECK(hipMemcpyAsync(cx.points.p, points, n * sizeof(hs_gpu_jacobian), hipMemcpyHostToDevice, st));
-    if ((r = msm_device_any(curve, cx, cx.scalars.p, cx.points.p, n, flags, cx.out32.p, cx.outjac.p, st)))
-        return r;
-    unsigned char o32[32];
-    hs_gpu_jacobian oj;
-    HS_CHECK(hipMemcpyAsync(o32, cx.out32.p, 32, hipMemcpyDeviceToHost, st));
-    HS_CHECK(hipMemcpyAsync(&oj, cx.outjac.p, sizeof oj, hipMemcpyDeviceToHost, st));
-    HS_CHECK(hipStreamSynchronize(st));
     if (out32)
-        std::memcpy(out32, o32, 32);
+        std::memcpy(out32, o, 32);
     if (result)
-        std::memcpy(result, &oj, sizeof oj);
+        std::memcpy(result, o + 32, sizeof(hs_gpu_jacobian));
     return 0;
+}
+
+// Registered CPU fallbacks of the void drop-ins, per curve.
+std::atomic<hs_gpu_cpu_msm_fn> g_fb_msm[2];
+std::atomic<hs_gpu_cpu_msm_fixed_fn> g_fb_fixed[2];
+std::atomic<uint64_t> g_fb_calls{0};
+
+[[noreturn]] void die(const char *what, int curve, int r)
+{
+    std::fprintf(stderr,
+                 "hs_gpu %s_%s failed (%d): %s; no CPU fallback is registered (hs_gpu_set_cpu_fallback), so the "
+                 "process stops here rather than return a wrong point\n",
+                 curve == 0 ? "helios" : "selene", what, r, g_err.c_str());
+    std::fflush(stderr);
+    std::abort();
+}
+
+void slot_flags(int curve, hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_jacobian *points,
+                size_t n, unsigned flags)
+{
+    const int r = msm_host(curve, result, nullptr, scalars, points, n, flags);
+    if (r == 0)
+        return;
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        die("msm_vartime", curve, r);
+    if (hs_gpu_cpu_msm_fn fb = g_fb_msm[curve].load())
+    {
+        g_fb_calls++;
+        fb(result, scalars, points, n);
+        return;
+    }
+    die("msm_vartime", curve, r);
 }
 
 void slot(int curve, hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_jacobian *points, size_t n)
 {
-    int r = msm_host(curve, result, nullptr, scalars, points, n, 0);
-    if (r != 0)
-    {
-        std::fprintf(stderr, "hs_gpu %s_msm_vartime failed (%d): %s\n", curve == 0 ? "helios" : "selene", r,
-                     g_err.c_str());
-        if (result)
-            set_identity(result);
-    }
+    slot_flags(curve, result, scalars, points, n, 0);
 }
 
 } // namespace
@@ -976,6 +1149,13 @@ constexpr size_t FIXED_TABLE_BUDGET = (size_t)9 << 30;
 
 int fixed_width(size_t n)
 {
+    // HS_GPU_FIXED_WIDTH forces a layout that fits (tests of every width at small n): 16 = comb
+    const int forced = env_int("HS_GPU_FIXED_WIDTH", 0);
+    if (forced == 16 && n * ((size_t)64 << 15) <= FIXED_TABLE_BUDGET)
+        return hs::FIXED_COMB16;
+    for (int wb : {14, 13, 12, 10, 8, 5})
+        if (forced == wb && n * ((size_t)64 << (wb - 1)) <= FIXED_TABLE_BUDGET)
+            return wb;
     if (n * ((size_t)64 << 15) <= FIXED_TABLE_BUDGET)
         return hs::FIXED_COMB16;
     for (int wb : {14, 13, 12, 10, 8})
@@ -1158,8 +1338,8 @@ int fixed_msm_host(hs_gpu_fixed &f, const unsigned char *scalars, size_t count, 
     int r;
     if ((r = ds.reserve(count * f.n * 32)) || (r = dout.reserve(count * (32 + sizeof(hs_gpu_jacobian)))))
     {
-        hipFree(ds.p);
-        hipFree(dout.p);
+        ds.release();
+        dout.release();
         return r;
     }
     unsigned char *o32 = dout.as<unsigned char>();
@@ -1173,6 +1353,17 @@ int fixed_msm_host(hs_gpu_fixed &f, const unsigned char *scalars, size_t count, 
         r = fail(HS_GPU_EDEVICE, "result copy failed");
     if (!r && results && hipMemcpyAsync(results, oj, count * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToHost, st) != hipSuccess)
         r = fail(HS_GPU_EDEVICE, "result copy failed");
+    if (flags & HS_GPU_ERASE)
+    {
+        zero_dev(ds.p, count * f.n * 32, st);
+        zero_dev(f.scal.p, f.scal.cap, st); // prepared scalars / column patterns
+        if (f.pip)
+            for (BatchGroup *g : f.pip->groups)
+            {
+                zero_dev(g->digits.p, g->digits.cap, st);
+                zero_dev(g->sorted.p, g->sorted.cap, st);
+            }
+    }
     if (hipStreamSynchronize(st) != hipSuccess && !r)
         r = fail(HS_GPU_EDEVICE, "fixed-base MSM failed");
     if (f.pip && method == HS_GPU_FIXED_PIPPENGER)
@@ -1180,8 +1371,8 @@ int fixed_msm_host(hs_gpu_fixed &f, const unsigned char *scalars, size_t count, 
         hs_gpu_batch_destroy(f.pip); // its descriptors point at the buffers freed below
         f.pip = nullptr;
     }
-    hipFree(ds.p);
-    hipFree(dout.p);
+    ds.release();
+    dout.release();
     return r;
 }
 
@@ -1213,6 +1404,7 @@ void slot_msm_fixed(int curve, hs_gpu_jacobian *result, const unsigned char *sca
             else
                 std::memcpy(&h[i * hs::FIXED_ENTRIES], tables[i], hs::FIXED_ENTRIES * sizeof(hs_gpu_affine));
         }
+        // the tables are public (multiples of the bases); the scalars are erased in fixed_msm_host
         DevBuf staged;
         hipStream_t st = ctx().stream;
         if (!r && !(r = staged.reserve(h.size() * sizeof(hs_gpu_affine))))
@@ -1229,19 +1421,22 @@ void slot_msm_fixed(int curve, hs_gpu_jacobian *result, const unsigned char *sca
                 else
                     hipLaunchKernelGGL(hs::k_fixed_tables_from_ref<hs::SeleneC>, dim3((entries + 255) / 256), dim3(256),
                                        0, st, staged.as<uint64_t>(), entries, f->tables.as<uint32_t>());
-                r = fixed_msm_host(*f, scalars, 1, HS_GPU_EXACT, HS_GPU_FIXED_COMB, nullptr, result);
+                r = fixed_msm_host(*f, scalars, 1, HS_GPU_EXACT | HS_GPU_ERASE, HS_GPU_FIXED_COMB, nullptr, result);
             }
-            hipFree(staged.p);
-            staged.p = nullptr;
+            staged.release();
         }
     }
     if (f)
         hs_gpu_fixed_destroy(f);
     if (r)
     {
-        std::fprintf(stderr, "hs_gpu %s_msm_fixed failed (%d): %s\n", curve == 0 ? "helios" : "selene", r, g_err.c_str());
-        if (result)
-            set_identity(result);
+        if (hs_gpu_cpu_msm_fixed_fn fb = (curve == HS_GPU_HELIOS || curve == HS_GPU_SELENE) ? g_fb_fixed[curve].load() : nullptr)
+        {
+            g_fb_calls++;
+            fb(result, scalars, (const struct hs_gpu_affine *const *)tables, n);
+            return;
+        }
+        die("msm_fixed", curve, r);
     }
 }
 
@@ -1264,7 +1459,11 @@ void slot_pedersen(int curve, hs_gpu_jacobian *result, const unsigned char *blin
         std::memcpy(s.data() + 32, values, 32 * n);
         std::memcpy(p.data() + 1, generators, n * sizeof(hs_gpu_jacobian));
     }
-    slot(curve, result, s.data(), p.data(), n + 1);
+    hs_gpu_jacobian r;
+    slot_flags(curve, &r, s.data(), p.data(), n + 1, HS_GPU_ERASE);
+    secure_erase(s.data(), s.size()); // blinding and values (helios_pedersen.h:85)
+    if (result)
+        *result = r;
 }
 
 } // namespace
@@ -1541,8 +1740,7 @@ struct TuneInputs
     ~TuneInputs()
     {
         for (DevBuf *d : {&u, &pts, &sc, &out})
-            if (d->p)
-                hipFree(d->p);
+            d->release();
     }
 };
 
@@ -1632,7 +1830,7 @@ int hs_gpu_server_create(const hs_gpu_server_config *cfg, hs_gpu_server **out)
     {
         bool ok = hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking) == hipSuccess &&
                   hipEventCreateWithFlags(&ln.done, hipEventDisableTiming) == hipSuccess &&
-                  hipMalloc((void **)&ln.arena.base, s->lane_bytes) == hipSuccess &&
+                  dev_malloc((void **)&ln.arena.base, s->lane_bytes) == 0 &&
                   hipHostMalloc((void **)&ln.pin_out, (size_t)s->max_batch * (32 + sizeof(hs_gpu_jacobian)), 0) ==
                       hipSuccess;
         ln.arena.cap = ok ? s->lane_bytes : 0;
@@ -1765,7 +1963,7 @@ void hs_gpu_server_destroy(hs_gpu_server *s)
         if (ln.st)
             hipStreamDestroy(ln.st);
         if (ln.arena.base)
-            hipFree(ln.arena.base);
+            dev_free(ln.arena.base, s->lane_bytes);
         if (ln.pin_out)
             hipHostFree(ln.pin_out);
     }
@@ -1831,6 +2029,43 @@ int hs_gpu_init(void)
     return cx.init();
 }
 
+int hs_gpu_set_cpu_fallback(int curve, hs_gpu_cpu_msm_fn msm, hs_gpu_cpu_msm_fixed_fn msm_fixed)
+{
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        return fail(HS_GPU_EINVAL, "unknown curve id");
+    g_fb_msm[curve].store(msm);
+    g_fb_fixed[curve].store(msm_fixed);
+    return 0;
+}
+
+uint64_t hs_gpu_fallback_count(void) { return g_fb_calls.load(); }
+
+int hs_gpu_set_device_limit(size_t bytes)
+{
+    g_dev_limit.store(bytes);
+    return 0;
+}
+
+size_t hs_gpu_device_bytes(void) { return g_dev_held.load(); }
+
+int hs_gpu_trim(void)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    if (!cx.ready)
+        return 0;
+    // every stream the cached workspaces serve has finished before they are freed
+    HS_CHECK(hipDeviceSynchronize());
+    for (auto &kv : cx.wss)
+        kv.second->release_all();
+    for (HostStage *h : cx.stages_all)
+        for (DevBuf *d : {&h->scalars, &h->points, &h->out})
+            d->release();
+    for (DevBuf *d : {&cx.scalars, &cx.points, &cx.out32, &cx.outjac, &cx.status, &cx.nbad, &cx.probe})
+        d->release();
+    return 0;
+}
+
 const char *hs_gpu_last_error(void) { return g_err.c_str(); }
 
 int hs_gpu_batch_create(const hs_gpu_msm_desc *descs, size_t count, unsigned flags, hs_gpu_batch **out)
@@ -1889,8 +2124,7 @@ void hs_gpu_batch_destroy(hs_gpu_batch *b)
         if (!b->in_arena)
             for (DevBuf *d : {&g->desc, &g->aff, &g->isid, &g->digits, &g->sorted, &g->offsets, &g->buckets, &g->seg,
                               &g->win, &g->total, &g->chunk_bucket, &g->bnd, &g->long_list, &g->long_count})
-                if (d->p)
-                    hipFree(d->p);
+                d->release();
         delete g;
     }
     if (!b->borrowed)
@@ -1902,8 +2136,8 @@ void hs_gpu_batch_destroy(hs_gpu_batch *b)
     }
     if (b->cap_stream)
         hipStreamDestroy(b->cap_stream);
-    if (b->empty.p && !b->in_arena)
-        hipFree(b->empty.p);
+    if (!b->in_arena)
+        b->empty.release();
     delete b;
 }
 
@@ -1911,10 +2145,19 @@ int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
 {
     if (count && !descs)
         return fail(HS_GPU_EINVAL, "null descriptors");
-    // stage every input in one device buffer, run the batch, copy the outputs back
+    // validate everything before any allocation
     size_t n_tot = 0;
     for (size_t i = 0; i < count; i++)
-        n_tot += descs[i].n;
+    {
+        const hs_gpu_msm_desc &d = descs[i];
+        if (d.curve != HS_GPU_HELIOS && d.curve != HS_GPU_SELENE)
+            return fail(HS_GPU_EINVAL, "bad curve in batch descriptor " + std::to_string(i));
+        if (d.n && (!d.scalars || !d.points))
+            return fail(HS_GPU_EINVAL, "null input in batch descriptor " + std::to_string(i));
+        if (d.n >= (1u << 24))
+            return fail(HS_GPU_EINVAL, "batch element " + std::to_string(i) + ": n >= 2^24 (the plain MSM takes larger sizes)");
+        n_tot += d.n;
+    }
     Ctx &cx = ctx();
     int r;
     {
@@ -1922,64 +2165,76 @@ int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
         if ((r = cx.init()))
             return r;
     }
+    // stage every input in one device buffer, run the batch, copy the outputs back; every exit
+    // below goes through the cleanup at the end
     DevBuf in, outb;
-    const size_t in_bytes = n_tot * (32 + sizeof(hs_gpu_jacobian));
-    if ((r = in.reserve(in_bytes + 64)) || (r = outb.reserve(count * (32 + sizeof(hs_gpu_jacobian)) + 64)))
-        return r;
-    std::vector<hs_gpu_msm_desc> dd(descs, descs + count);
-    unsigned char *sc_base = in.as<unsigned char>(), *pt_base = sc_base + n_tot * 32;
-    unsigned char *o32_base = outb.as<unsigned char>(), *oj_base = o32_base + count * 32;
-    hipStream_t st = cx.stream;
-    size_t off = 0;
-    for (size_t i = 0; i < count; i++)
-    {
-        const size_t n = descs[i].n;
-        if (n)
-        {
-            HS_CHECK(hipMemcpyAsync(sc_base + off * 32, descs[i].scalars, n * 32, hipMemcpyHostToDevice, st));
-            HS_CHECK(hipMemcpyAsync(pt_base + off * sizeof(hs_gpu_jacobian), descs[i].points, n * sizeof(hs_gpu_jacobian),
-                                    hipMemcpyHostToDevice, st));
-        }
-        dd[i].scalars = sc_base + off * 32;
-        dd[i].points = (const hs_gpu_jacobian *)(pt_base + off * sizeof(hs_gpu_jacobian));
-        dd[i].out32 = o32_base + i * 32;
-        dd[i].result = (hs_gpu_jacobian *)(oj_base + i * sizeof(hs_gpu_jacobian));
-        off += n;
-    }
-    HS_CHECK(hipStreamSynchronize(st));
     hs_gpu_batch *b = nullptr;
-    if ((r = batch_create(&b, dd.data(), count, flags)))
-        return r;
-    r = batch_enqueue(*b, st);
-    if (!r)
+    hipStream_t st = cx.stream;
+    const size_t in_bytes = n_tot * (32 + sizeof(hs_gpu_jacobian));
+    std::vector<hs_gpu_msm_desc> dd(descs, descs + count);
+    if (!(r = in.reserve(in_bytes + 64)) && !(r = outb.reserve(count * (32 + sizeof(hs_gpu_jacobian)) + 64)))
     {
-        std::vector<unsigned char> o32(count * 32);
-        std::vector<hs_gpu_jacobian> oj(count);
-        hipError_t e1 = hipMemcpyAsync(o32.data(), o32_base, count * 32, hipMemcpyDeviceToHost, st);
-        hipError_t e2 = hipMemcpyAsync(oj.data(), oj_base, count * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToHost, st);
-        hipError_t e3 = hipStreamSynchronize(st);
-        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-            r = fail(HS_GPU_EDEVICE, "batch result copy failed");
+        unsigned char *sc_base = in.as<unsigned char>(), *pt_base = sc_base + n_tot * 32;
+        unsigned char *o32_base = outb.as<unsigned char>(), *oj_base = o32_base + count * 32;
+        size_t off = 0;
         for (size_t i = 0; i < count && !r; i++)
         {
-            if (descs[i].n == 0)
+            const size_t n = descs[i].n;
+            if (n && (hipMemcpyAsync(sc_base + off * 32, descs[i].scalars, n * 32, hipMemcpyHostToDevice, st) != hipSuccess ||
+                      hipMemcpyAsync(pt_base + off * sizeof(hs_gpu_jacobian), descs[i].points, n * sizeof(hs_gpu_jacobian),
+                                     hipMemcpyHostToDevice, st) != hipSuccess))
+                r = fail(HS_GPU_EDEVICE, "batch input upload failed");
+            dd[i].scalars = sc_base + off * 32;
+            dd[i].points = (const hs_gpu_jacobian *)(pt_base + off * sizeof(hs_gpu_jacobian));
+            dd[i].out32 = o32_base + i * 32;
+            dd[i].result = (hs_gpu_jacobian *)(oj_base + i * sizeof(hs_gpu_jacobian));
+            off += n;
+        }
+        if (!r && hipStreamSynchronize(st) != hipSuccess)
+            r = fail(HS_GPU_EDEVICE, "batch input upload failed");
+        if (!r)
+            r = batch_create(&b, dd.data(), count, flags);
+        if (!r)
+            r = batch_enqueue(*b, st);
+        if (!r)
+        {
+            std::vector<unsigned char> o32(count * 32);
+            std::vector<hs_gpu_jacobian> oj(count);
+            if (flags & HS_GPU_ERASE)
             {
-                if (descs[i].out32)
-                    std::memset(descs[i].out32, 0, 32);
-                if (descs[i].result)
-                    set_identity(descs[i].result);
-                continue;
+                zero_dev(sc_base, n_tot * 32, st);
+                for (BatchGroup *g : b->groups)
+                {
+                    zero_dev(g->digits.p, g->digits.cap, st);
+                    zero_dev(g->sorted.p, g->sorted.cap, st);
+                }
             }
-            if (descs[i].out32)
-                std::memcpy(descs[i].out32, &o32[i * 32], 32);
-            if (descs[i].result)
-                descs[i].result[0] = oj[i];
+            hipError_t e1 = hipMemcpyAsync(o32.data(), o32_base, count * 32, hipMemcpyDeviceToHost, st);
+            hipError_t e2 = hipMemcpyAsync(oj.data(), oj_base, count * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToHost, st);
+            hipError_t e3 = hipStreamSynchronize(st);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+                r = fail(HS_GPU_EDEVICE, "batch result copy failed");
+            for (size_t i = 0; i < count && !r; i++)
+            {
+                if (descs[i].n == 0)
+                {
+                    if (descs[i].out32)
+                        std::memset(descs[i].out32, 0, 32);
+                    if (descs[i].result)
+                        set_identity(descs[i].result);
+                    continue;
+                }
+                if (descs[i].out32)
+                    std::memcpy(descs[i].out32, &o32[i * 32], 32);
+                if (descs[i].result)
+                    descs[i].result[0] = oj[i];
+            }
         }
     }
+    hipStreamSynchronize(st);
     hs_gpu_batch_destroy(b);
-    hipFree(in.p);
-    hipFree(outb.p);
-    in.p = outb.p = nullptr;
+    in.release();
+    outb.release();
     return r;
 }
 
@@ -2320,7 +2575,9 @@ int hs_gpu_pedersen_commit_batch(hs_gpu_fixed *f, const unsigned char *blindings
             std::memcpy(&rows[k * f->n * 32 + 32], values + k * nv * 32, nv * 32);
     }
     std::lock_guard<std::mutex> lk(f->mu);
-    return fixed_msm_host(*f, rows.data(), count, flags, method, out32, results);
+    const int r = fixed_msm_host(*f, rows.data(), count, flags | HS_GPU_ERASE, method, out32, results);
+    secure_erase(rows.data(), rows.size());
+    return r;
 }
 
 void hs_gpu_fixed_destroy(hs_gpu_fixed *f)
@@ -2330,8 +2587,7 @@ void hs_gpu_fixed_destroy(hs_gpu_fixed *f)
     if (f->pip)
         hs_gpu_batch_destroy(f->pip);
     for (DevBuf *d : {&f->tables, &f->points, &f->scal, &f->sums})
-        if (d->p)
-            hipFree(d->p);
+        d->release();
     delete f;
 }
 
@@ -2359,7 +2615,7 @@ int hs_gpu_fixed_precompute(int curve, const hs_gpu_jacobian *points, size_t n, 
             r = fail(HS_GPU_EDEVICE, "table copy failed");
         if (!r && hipStreamSynchronize(st) != hipSuccess)
             r = fail(HS_GPU_EDEVICE, "table build failed");
-        hipFree(ref.p);
+        ref.release();
     }
     hs_gpu_fixed_destroy(f);
     return r;

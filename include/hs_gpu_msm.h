@@ -22,8 +22,9 @@
  *
  * Errors: functions returning int give 0 on success or a negative HS_GPU_E* code;
  * hs_gpu_last_error() describes the most recent failure of the calling thread.
- * Threading: every entry point may be called from several host threads; calls on one device are
- * serialised internally (one in-flight plain MSM per device context).
+ * Threading: every entry point may be called from several host threads.  Host-pointer MSMs from
+ * concurrent callers each take a staging slot of their own (stream, upload buffers, workspace),
+ * so they overlap on the device; only their launch sequences are serialised.
  * Sizes: a plain MSM of 2^24 points or more runs as chunks of 2^23 points whose sums are added on
  * the device (same results); batch elements, server requests and fixed-base sets are limited to
  * fewer than 2^24 points each.
@@ -43,6 +44,9 @@ extern "C" {
 
 /* flags */
 #define HS_GPU_EXACT 1u /* exact sum; do not reproduce the reference's window-8 carry drop */
+#define HS_GPU_ERASE 2u /* zero the device copies of the scalars and of their digits after use (the
+                           Pedersen and msm_fixed drop-ins always do; helios_pedersen.h:85,
+                           helios_msm_fixed.h:151 erase theirs) */
 
 /* status codes */
 #define HS_GPU_OK 0
@@ -66,12 +70,27 @@ typedef struct hs_gpu_jacobian
  * (helios/include/helios_msm_vartime.h:43-72; slot types include/helioselene_dispatch.h:53,56;
  * backends helios_msm_vartime_{x64,avx2,ifma}, src/dispatch.cpp:133-179) and the selene_ twin.
  * n == 0 gives the identity (helios_msm_vartime.cpp:402-406).  The result is written last, so it
- * may alias the inputs.  On a device failure the message is printed to stderr, the result is set
- * to the identity and hs_gpu_last_error() reports the cause (use hs_gpu_msm_ex for a status). */
+ * may alias the inputs.  A device failure (no gfx950 device, a refused allocation, a HIP error)
+ * never yields a wrong point: the slot calls the CPU function registered with
+ * hs_gpu_set_cpu_fallback (the reference's own backend, e.g. helios_msm_vartime_x64, which
+ * returns the reference's bytes), and with none registered it prints the cause and aborts, as the
+ * reference's std::bad_alloc would end a C caller.  hs_gpu_msm_ex returns a status instead. */
 void hs_gpu_helios_msm_vartime(hs_gpu_jacobian *result, const unsigned char *scalars,
                                const hs_gpu_jacobian *points, size_t n);
 void hs_gpu_selene_msm_vartime(hs_gpu_jacobian *result, const unsigned char *scalars,
                                const hs_gpu_jacobian *points, size_t n);
+
+/* CPU fallbacks of the void drop-ins (no reference counterpart: the reference has no device to
+ * fail).  msm has the slot signature; msm_fixed the helios_msm_fixed signature over hs_gpu_affine
+ * tables.  Either may be NULL (no fallback for that entry point).  hs_gpu_fallback_count()
+ * reports how many drop-in calls were answered by a fallback. */
+typedef void (*hs_gpu_cpu_msm_fn)(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_jacobian *points,
+                                  size_t n);
+struct hs_gpu_affine;
+typedef void (*hs_gpu_cpu_msm_fixed_fn)(hs_gpu_jacobian *result, const unsigned char *scalars,
+                                        const struct hs_gpu_affine *const *tables, size_t n);
+int hs_gpu_set_cpu_fallback(int curve, hs_gpu_cpu_msm_fn msm, hs_gpu_cpu_msm_fixed_fn msm_fixed);
+uint64_t hs_gpu_fallback_count(void);
 
 /* Status-returning form of the slot, with flags.  Either output may be NULL. */
 int hs_gpu_msm_ex(int curve, hs_gpu_jacobian *result, unsigned char out32[32], const unsigned char *scalars,
@@ -146,8 +165,12 @@ void hs_gpu_selene_pedersen_commit(hs_gpu_jacobian *result, const unsigned char 
 int hs_gpu_fixed_precompute(int curve, const hs_gpu_jacobian *points, size_t n, hs_gpu_affine *tables);
 
 /* Drop-in for helios_msm_fixed / selene_msm_fixed (helios/include/helios_msm_fixed.h:60-61):
- * sum scalars[i] * P_i from the callers' 16-entry tables.  Exact sum (the reference's mixed
- * additions assume the accumulator never equals +-the table entry; this one has no such case). */
+ * sum scalars[i] * P_i from the callers' 16-entry tables.  Always the exact sum.  This departs
+ * from the reference where its madd-2007-bl (helios_msm_fixed.h:125-136) meets an accumulator
+ * equal to the table entry it adds: the reference then drops the pair and continues from the
+ * identity (for example bases [P, P] with scalars [s, s] give the identity there, 2 s P here;
+ * tests/golden/workload_runs.json holds the reference's bytes for such inputs).  Random inputs
+ * never meet the case. */
 void hs_gpu_helios_msm_fixed(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_affine *const *tables,
                              size_t n);
 void hs_gpu_selene_msm_fixed(hs_gpu_jacobian *result, const unsigned char *scalars, const hs_gpu_affine *const *tables,
@@ -263,6 +286,17 @@ int hs_gpu_probe_int_mul_peak(double *products_per_second);
  * window sums, 1 segment sums, 2 bucket sums (as 32-byte encodings), 3 sorted bucket entries,
  * 4 bucket offsets (uint32).  *count receives the number of items written. */
 int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t *count);
+
+/* ---------------------------------------------------------------- device memory
+ * No reference counterpart.  The library keeps its workspaces between calls (no allocation on the
+ * steady-state path).  hs_gpu_set_device_limit caps the device bytes it may hold (0 = no cap; the
+ * environment variable HS_GPU_DEVICE_LIMIT sets the initial cap); an allocation past the cap fails
+ * with HS_GPU_ENOMEM.  hs_gpu_device_bytes reports the bytes held; hs_gpu_trim waits for the
+ * library's streams and frees every cached workspace (resident sets, batch plans and servers keep
+ * theirs). */
+int hs_gpu_set_device_limit(size_t bytes);
+size_t hs_gpu_device_bytes(void);
+int hs_gpu_trim(void);
 
 /* ---------------------------------------------------------------- library */
 int hs_gpu_init(void);              /* selects device 0 (or the current device); idempotent */

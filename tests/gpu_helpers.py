@@ -136,3 +136,42 @@ def ref_sswu_bases(c, seed, n, dup_frac=0.0, id_frac=0.0):
         i = int(rng.integers(0, n))
         pb[32 * i:32 * i + 32] = bytes(32)
     return bytes(pb)
+
+
+# ---------------------------------------------------------------- CPU fallback for the drop-ins
+def struct_to_bytes(c, words):
+    """reference Jacobian struct (15 uint64, any Z, lazy limbs) -> compressed encoding (Python ints)"""
+    m = O.FIELD[c]
+    X, Y, Z = (sum(int(words[5 * k + j]) << (51 * j) for j in range(5)) % m for k in range(3))
+    if Z == 0:
+        return bytes(32)
+    zi = pow(Z, -1, m)
+    x, y = X * zi * zi % m, Y * zi * zi * zi % m
+    b = bytearray(x.to_bytes(32, "little"))
+    b[31] |= (y & 1) << 7
+    return bytes(b)
+
+
+def bytes_to_struct(c, p32):
+    xy = decompress(c, p32)
+    out = np.zeros(15, dtype=np.uint64)
+    if xy is None:
+        out[0] = out[5] = 1
+        return out
+    m51 = (1 << 51) - 1
+    for k, v in enumerate((xy[0], xy[1], 1)):
+        out[5 * k:5 * k + 5] = [(v >> (51 * j)) & m51 for j in range(5)]
+    return out
+
+
+def oracle_fallback(c, calls):
+    """A CPU MSM with the drop-in slot's signature, computed by the oracle (test infrastructure):
+    stands in for the reference backend a caller registers with hs_gpu_set_cpu_fallback."""
+    def fb(res, sc, pts, n):
+        calls.append(n)
+        s = ctypes.string_at(sc, 32 * n)
+        w = np.frombuffer(ctypes.string_at(pts, 120 * n), dtype=np.uint64).reshape(n, 15)
+        pb = b"".join(struct_to_bytes(c, w[i]) for i in range(n))
+        out = bytes_to_struct(c, O.msm(c, s, pb, n))
+        ctypes.memmove(res, out.ctypes.data, 120)
+    return fb

@@ -184,3 +184,46 @@ def test_fixed_set_windowed_tables(c):
         want = O.msm_naive(c, rows[k], bytes(pb), n)
         assert comb[k] == want and pip[k] == want, k
         assert GH.jac_to_bytes(c, jac[k]) == want
+
+
+@pytest.mark.parametrize("c", [0, 1])
+@pytest.mark.parametrize("width", [16, 14, 13, 12, 10, 8, 5])
+def test_fixed_set_every_table_width(c, width, monkeypatch):
+    """Every resident-set layout (comb = 16, windows of 14..5 bits), forced at a small n: an identity
+    base, a duplicated base and edge scalars, default and exact modes, against the oracle and the
+    Pippenger method."""
+    monkeypatch.setenv("HS_GPU_FIXED_WIDTH", str(width))
+    n, K = 70, 3
+    pb = bytearray(O.gen_sswu_bases(c, 800 + width, n))
+    pb[32 * 3:32 * 4] = bytes(32)
+    pb[32 * 9:32 * 10] = pb[32 * 4:32 * 5]
+    pts = GH.to_jac_array(c, bytes(pb), n, rnd=random.Random(width))
+    rows = [O.gen_scalars(c, 810 + k + width, n, edge_frac=0.2) for k in range(K)]
+    S = np.frombuffer(b"".join(rows), np.uint8).reshape(K, n, 32)
+    fs = hs.FixedSet(c, pts)
+    comb, _ = fs.msm(S, method=hs.FIXED_COMB)
+    comb_x, _ = fs.msm(S, flags=hs.EXACT, method=hs.FIXED_COMB)
+    pip, _ = fs.msm(S, flags=hs.EXACT, method=hs.FIXED_PIPPENGER)
+    fs.close()
+    for k in range(K):
+        want = O.msm_naive(c, rows[k], bytes(pb), n)
+        assert comb[k] == O.msm(c, rows[k], bytes(pb), n) == want
+        assert comb_x[k] == want and pip[k] == want, k
+
+
+@pytest.mark.parametrize("c", [0, 1])
+def test_msm_fixed_degenerate_case_is_exact(c):
+    """Repeated (base, scalar) pairs, where the reference's msm_fixed meets an accumulator equal to
+    the table entry and drops the pair: the drop-in returns the exact sum (the documented
+    departure); the reference's own bytes are in the fixture."""
+    import json
+    import os
+
+    runs = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "workload_runs.json")))
+    for r in [x for x in runs if x["kind"] == "fixed_degenerate" and x["curve"] == O.CURVES[c]]:
+        pb = b"".join(H(p) for p in r["points"])
+        sb = b"".join(H(s) for s in r["scalars"])
+        n = len(r["points"])
+        tab = hs.fixed_precompute(c, GH.to_jac_array(c, pb, n))
+        got = GH.jac_to_bytes(c, hs.msm_fixed(c, GH.scalars_np(sb, n), tab))
+        assert got.hex() == r["exact_result"] and got.hex() != r["ref_result"], r["label"]

@@ -116,3 +116,83 @@ def test_device_inputs():
     for want, t in ts:
         assert srv.wait(t)[0] == want
     srv.close()
+
+
+def test_full_size_range_staggered():
+    """Requests log-uniform over the whole served range [32, 2^18] (the plain workload's
+    generator), submitted from two threads with staggered gaps while others run: each result
+    equals the plain MSM of the same request, and the three fixture requests equal the compiled
+    reference's bytes (tests/golden/workload_runs.json)."""
+    import json
+    import os
+
+    from helioselene_amd import workloads as W
+
+    runs = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "workload_runs.json")))
+    fixed = [r for r in runs if r["kind"] == "plain" and r["seed"] in (7, 8, 9)]
+    rng = np.random.default_rng(21)
+    reqs = []
+    for r in fixed:
+        c = O.CURVES.index(r["curve"])
+        w = W.plain_inputs(c, r["n"], r["seed"])
+        pts = W.apply_dups_and_identities(hs.map_to_curve(c, w["u"].tobytes())[0], w)
+        reqs.append((c, w["scalars"], pts, r["ref_result"]))
+    for k in range(21):
+        c = int(rng.integers(0, 2))
+        n = int(np.exp(rng.uniform(np.log(32), np.log(1 << 18))))
+        w = W.plain_inputs(c, n, 1000 + k)
+        pts = W.apply_dups_and_identities(hs.map_to_curve(c, w["u"].tobytes())[0], w)
+        reqs.append((c, w["scalars"], pts, None))
+    order = rng.permutation(len(reqs))
+    alone = [hs.msm(c, sc, pts)[0] for c, sc, pts, _ in reqs]
+    for (_, _, _, ref), a in zip(reqs, alone):
+        if ref is not None:
+            assert a.hex() == ref
+    srv = hs.Server(device_bytes=12 << 30, lanes=4)
+    results = [None] * len(reqs)
+
+    def client(part, seed):
+        g = np.random.default_rng(seed)
+        tickets = []
+        for i in part:
+            c, sc, pts, _ = reqs[i]
+            tickets.append((i, srv.submit(c, sc, pts)))
+            time.sleep(float(g.uniform(0, 0.004)))
+        for i, t in tickets:
+            results[i] = srv.wait(t)[0]
+
+    th = [threading.Thread(target=client, args=(list(order[j::2]), 50 + j)) for j in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    st = srv.stats()
+    srv.close()
+    assert results == alone
+    assert st["completed"] == len(reqs)
+
+
+def test_tuned_groups_with_mixed_run_lengths_fit_their_lane(tmp_path):
+    """A tune table whose entries for one window width differ in run length L and segment size G
+    (sizes 2^10 and 2^11 both at c = 8): a lane filled close to its budget with both sizes still
+    plans its batch inside the admitted footprint, and every result equals the plain MSM's."""
+    f = tmp_path / "tune.txt"
+    f.write_text("1 0 10 8 4 4 0.1\n1 0 11 8 64 16 0.1\n1 1 10 8 4 4 0.1\n1 1 11 8 64 16 0.1\n")
+    hs.tune_load(str(f))
+    try:
+        rng = np.random.default_rng(31)
+        reqs = []
+        for k in range(24):
+            c = k % 2
+            n = 1024 if k % 3 else 2048
+            sc = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+            sc[:, 31] &= 0x7F
+            pts = hs.map_to_curve(c, rng.integers(0, 256, size=n * 32, dtype=np.uint8).tobytes())[0]
+            reqs.append((c, sc, pts))
+        srv = hs.Server(device_bytes=2 * (10 << 20), lanes=2, max_batch=64)
+        ts = [srv.submit(c, sc, pts) for c, sc, pts in reqs]
+        got = [srv.wait(t)[0] for t in ts]
+        srv.close()
+        assert got == [hs.msm(c, sc, pts)[0] for c, sc, pts in reqs]
+    finally:
+        hs.tune_clear()
