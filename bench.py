@@ -23,8 +23,16 @@ The same line carries:
   * `serving`: Poisson request stream through the native server against plain per-request
     launches, p50/p99 latency and saturated throughput;
   * `host_slot_points_per_s`: the drop-in slot with host buffers (PCIe upload included);
+  * `parity_vs_reference`: the timed steps' own 32-byte results against the compiled reference's
+    bytes on the same inputs (tests/golden/workload_runs.json);
+  * `single_msm`: one MSM at a time per curve with the scalar upload and result download timed;
+  * `gate`: the whole single MSM's fraction of the INT32 roofline, and the same over the steps;
+  * `roofline.hbm_phases`: achieved HBM bandwidth of the sort/scatter and gather phases;
+  * `config1_helios_4096`: Helios n = 4096 on k*G points, GPU and reference CPU;
+  * `fcmpp_verify`: the FCMP++ verification MSM pairs (Z != 1 bases), GPU and reference CPU;
   * `cpu_baseline`: the reference's own CPU MSM (compiled from its sources into
-    oracle/_ref/libhsref.so) on a bounded sample of the same inputs, one thread.
+    oracle/_ref/libhsref.so) on a bounded sample of the same inputs, one pinned thread, median
+    of three runs, with the CPU model.
 """
 import argparse
 import ctypes
@@ -80,34 +88,70 @@ def measured_traffic(n):
 
 
 def make_inputs(hs, torch, curve, n, seed, dev):
-    import numpy as np
+    """The plain workload of helioselene_amd/workloads.py (the generator the parity tests and the
+    reference fixtures use) with the bases mapped by the library's SSWU kernel on the device."""
+    from helioselene_amd import workloads as W
 
-    rng = np.random.default_rng(seed)
-    order = hs.ORDER[curve]
-    u = torch.from_numpy(rng.integers(0, 256, size=(n, 32), dtype=np.uint8)).to(dev)
+    w = W.plain_inputs(curve, n, seed)
+    u = torch.from_numpy(w["u"]).to(dev)
     pts = torch.empty((n, 15), dtype=torch.int64, device=dev)
     hs.map_to_curve_device(curve, u.data_ptr(), n, pts.data_ptr(), None, torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
-    ndup, nid = n // 200, n // 1000
-    dst = torch.from_numpy(rng.choice(n, size=ndup + nid, replace=False)).to(dev)
-    src = torch.from_numpy(rng.integers(0, n, size=ndup)).to(dev)
-    pts[dst[:ndup]] = pts[src]
-    ident = torch.zeros(15, dtype=torch.int64, device=dev)
-    ident[0] = 1
-    ident[5] = 1
-    pts[dst[ndup:]] = ident
-    # scalars: uniform in [0, 2^255); values >= order (probability ~2^-128) are reduced
-    sc = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
-    sc[:, 31] &= 0x7F
-    top = sc[:, 16:].view(np.uint64)
-    for i in np.nonzero((top == np.uint64(0xFFFFFFFFFFFFFFFF)).all(axis=1))[0]:
-        v = int.from_bytes(sc[i].tobytes(), "little") % order
-        sc[i] = np.frombuffer(v.to_bytes(32, "little"), dtype=np.uint8)
-    k = n // 100
-    for j, i in enumerate(rng.choice(n, size=k, replace=False)):
-        v = (0, 1, order - 1, int(rng.integers(2, 256)))[j % 4]
-        sc[i] = np.frombuffer(v.to_bytes(32, "little"), dtype=np.uint8)
-    return torch.from_numpy(sc).to(dev), pts
+    W.apply_dups_and_identities(pts, w)
+    return torch.from_numpy(w["scalars"]).to(dev), pts
+
+
+def reference_results(kind):
+    """{(curve name, n, seed or batch): 32-byte hex} of the compiled reference on these workloads
+    (tests/golden/workload_runs.json, written by tests/golden/make_workload_fixtures.py)."""
+    path = os.path.join(ROOT, "tests", "golden", "workload_runs.json")
+    if not os.path.exists(path):
+        return {}
+    out = {}
+    for r in json.load(open(path)):
+        if r["kind"] == kind:
+            out[(r["curve"], r["n"], r.get("seed", r.get("batch")))] = r["ref_result"]
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_single(hs, torch, inputs, n, args, st):
+    """One MSM at a time per curve, the way a caller holding resident bases uses it: the scalars
+    go from pinned host memory to HBM, the MSM runs, the 32-byte result comes back; all inside the
+    timed region (bases resident, not timed)."""
+    import numpy as np
+
+    res = {}
+    for c, cv in enumerate(("helios", "selene")):
+        sc, pts = inputs[c]
+        host = sc.cpu().pin_memory()
+        dsc = torch.empty_like(sc)
+        o = torch.zeros(8, dtype=torch.int32, device=sc.device)
+        oh = torch.zeros(8, dtype=torch.int32).pin_memory()
+        lat = []
+        with torch.cuda.stream(st):
+            for i in range(2 + args.single_steps):
+                t0 = time.perf_counter()
+                dsc.copy_(host, non_blocking=True)
+                hs.msm_device(c, dsc.data_ptr(), pts.data_ptr(), n, o.data_ptr(), None, 0, st.cuda_stream)
+                oh.copy_(o, non_blocking=True)
+                st.synchronize()
+                lat.append(time.perf_counter() - t0)
+        lat = np.array(lat[2:]) * 1e3
+        res[cv] = {"ms_p50": float(np.percentile(lat, 50)), "ms_p99": float(np.percentile(lat, 99)),
+                   "points_per_s": n / (float(np.median(lat)) * 1e-3), "result": oh.numpy().tobytes().hex()}
+    res["workload"] = (f"one plain MSM at a time of n={n} per curve: scalar upload ({32 * n} B from pinned host "
+                       f"memory) + MSM + result download timed, bases resident; {args.single_steps} runs per curve")
+    return res
 
 
 def make_mix(hs, torch, dev, seed, per_curve=256, lo=64, hi=1 << 14):
@@ -138,6 +182,145 @@ def make_mix(hs, torch, dev, seed, per_curve=256, lo=64, hi=1 << 14):
                       out[k].data_ptr(), None))
         off[c] += n
     return items, out, (sc, pts), sizes
+
+
+def ref_lib():
+    lib = os.path.join(ROOT, "oracle", "_ref", "libhsref.so")
+    if not os.path.exists(lib):
+        return None
+    L = ctypes.CDLL(lib)
+    L.hsref_init.restype = ctypes.c_uint
+    L.hsref_init()
+    for cv in ("helios", "selene"):
+        f = getattr(L, f"hsref_{cv}_time_msm")
+        f.restype = ctypes.c_double
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+        getattr(L, f"hsref_{cv}_msm_jac_bytes").argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                            ctypes.c_size_t]
+    return L
+
+
+def cpu_time_median(L, curve, sc, pts, runs=3):
+    """Median wall seconds of `runs` single-threaded reference MSMs (one pinned core)."""
+    import numpy as np
+
+    f = getattr(L, ("hsref_helios_time_msm", "hsref_selene_time_msm")[curve])
+    s = np.ascontiguousarray(sc, dtype=np.uint8)
+    p = np.ascontiguousarray(pts).view(np.uint64)
+    return float(np.median([f(s.ctypes.data, p.ctypes.data, p.shape[0], 1, 1) for _ in range(runs)]))
+
+
+def cpu_ref_bytes(L, curve, sc, pts):
+    import numpy as np
+
+    out = ctypes.create_string_buffer(32)
+    s = np.ascontiguousarray(sc, dtype=np.uint8)
+    p = np.ascontiguousarray(pts).view(np.uint64)
+    getattr(L, f"hsref_{('helios', 'selene')[curve]}_msm_jac_bytes")(out, s.ctypes.data, p.ctypes.data, p.shape[0])
+    return out.raw.hex()
+
+
+def run_config1(hs, torch, args, st, L):
+    """Helios n = 4096 on points k_i G (k_i, s_i = 64 seeded bytes reduced mod the order, the
+    reference fuzz recipe): the GPU MSM (inputs resident) and the reference CPU MSM on the same
+    inputs.  The bases are computed on the GPU as 4096 one-term MSMs over the resident set {G}."""
+    import numpy as np
+
+    from helioselene_amd import workloads as W
+
+    n, order = 4096, hs.ORDER[0]
+    rng = np.random.default_rng(args.seed + 4000)
+
+    def reduced():
+        raw = rng.integers(0, 256, size=(n, 64), dtype=np.uint8)
+        vals = [int.from_bytes(raw[i].tobytes(), "little") % order for i in range(n)]
+        return np.frombuffer(b"".join(v.to_bytes(32, "little") for v in vals), dtype=np.uint8).reshape(n, 32).copy()
+
+    k, sc = reduced(), reduced()
+    g = hs.FixedSet(0, W.jacobian_struct(0, *W.GEN[0], 1).reshape(1, 15))
+    _, pts = g.msm(k.reshape(n, 1, 32), flags=hs.EXACT)
+    g.close()
+    dev = torch.device("cuda:0")
+    dsc, dpt = torch.from_numpy(sc).to(dev), torch.from_numpy(pts.view(np.int64)).to(dev)
+    o = torch.zeros(8, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    lat = []
+    with torch.cuda.stream(st):
+        for i in range(3 + args.single_steps):
+            t0 = time.perf_counter()
+            hs.msm_device(0, dsc.data_ptr(), dpt.data_ptr(), n, o.data_ptr(), None, 0, st.cuda_stream)
+            st.synchronize()
+            lat.append(time.perf_counter() - t0)
+    gpu_ms = float(np.median(lat[3:])) * 1e3
+    got = o.cpu().numpy().tobytes().hex()
+    res = {"workload": "Helios MSM n=4096, points k_i*G, scalars uniform mod order (64 seeded bytes reduced)",
+           "gpu_ms": gpu_ms, "gpu_points_per_s": n / (gpu_ms * 1e-3)}
+    if L is not None:
+        t = cpu_time_median(L, 0, sc, pts)
+        res.update({"cpu_ms": t * 1e3, "cpu_points_per_s": n / t, "cpu_cores": 1,
+                    "cpu_kind": "reference (oracle/_ref, helios_msm_vartime via helioselene_init dispatch), median of 3",
+                    "bit_exact_vs_reference": got == cpu_ref_bytes(L, 0, sc, pts)})
+    return res
+
+
+def run_fcmpp(hs, torch, args, st, L):
+    """FCMP++ verification MSMs (src/benchmark_fcmpp.cpp:347-365 of the reference): Selene 522 + 80 b
+    and Helios 265 + 80 b for b proofs, on doubling-chain bases (Z != 1: every point goes through the
+    ingest's inversion) with the benchmark's scalars.  Per b: the two MSMs as plain launches and as
+    one batch (captured graph), latency per verification, against the reference CPU MSMs."""
+    import numpy as np
+
+    from helioselene_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    ref = reference_results("fcmpp")
+    out = {"workload": "per verification: Selene 522+80b and Helios 265+80b points, doubling-chain bases",
+           "batches": {}}
+    for b in W.FCMPP_BATCHES:
+        shapes = [(1, W.FCMPP_FIXED[1] + W.FCMPP_PER_PROOF * b), (0, W.FCMPP_FIXED[0] + W.FCMPP_PER_PROOF * b)]
+        host, dev_in, items = [], [], []
+        res_t = torch.zeros((2, 8), dtype=torch.int32, device=dev)
+        for k, (c, n) in enumerate(shapes):
+            pts, sc = W.doubling_chain(c, n), W.fcmpp_scalars(n)
+            host.append((c, sc, pts))
+            ds, dp = torch.from_numpy(sc).to(dev), torch.from_numpy(pts.view(np.int64)).to(dev)
+            dev_in.append((ds, dp))
+            items.append((c, n, ds.data_ptr(), dp.data_ptr(), res_t[k].data_ptr(), None))
+        torch.cuda.synchronize()
+        sh = st.cuda_stream
+
+        def plain():
+            for (c, n, ds, dp, do, _) in items:
+                hs.msm_device(c, ds, dp, n, do, None, 0, sh)
+
+        def timed(fn, reps):
+            lat = []
+            with torch.cuda.stream(st):
+                for i in range(3 + reps):
+                    t0 = time.perf_counter()
+                    fn()
+                    st.synchronize()
+                    lat.append(time.perf_counter() - t0)
+            return float(np.median(lat[3:])) * 1e3
+
+        t_plain = timed(plain, args.single_steps)
+        ok_plain = [res_t[k].cpu().numpy().tobytes().hex() for k in range(2)]
+        bt = hs.Batch(items)
+        bt.capture()
+        res_t.zero_()
+        t_batch = timed(lambda: bt.launch(sh), args.single_steps)
+        ok_batch = [res_t[k].cpu().numpy().tobytes().hex() for k in range(2)]
+        bt.close()
+        want = [ref.get((("helios", "selene")[c], n, b)) for c, n in shapes]
+        npts = sum(n for _, n in shapes)
+        rec = {"points": npts, "gpu_plain_ms": t_plain, "gpu_batched_ms": t_batch,
+               "gpu_points_per_s": npts / (min(t_plain, t_batch) * 1e-3),
+               "bit_exact_vs_reference": ok_plain == want and ok_batch == want}
+        if L is not None:
+            tc = sum(cpu_time_median(L, c, sc, pts) for c, sc, pts in host)
+            rec.update({"cpu_ms": tc * 1e3, "cpu_points_per_s": npts / tc})
+        out["batches"][str(b)] = rec
+    return out
 
 
 def run_mix(hs, torch, args):
@@ -241,11 +424,43 @@ def run_pedersen(hs, torch, args, count=1024, m=4096):
                 res[f"{['helios', 'selene'][c]}_{name}_ms"] = tm * 1e3
                 res[f"{['helios', 'selene'][c]}_{name}_points_per_s"] = count * (m + 1) / tm
                 outs[(c, name)] = out.cpu().numpy()
-            res[f"{['helios', 'selene'][c]}_methods_agree"] = bool((outs[(c, "comb")] == outs[(c, "pippenger")]).all())
             fs.close()
+            # the width-5 affine tables of the workload as written ([1P..16P] per generator, 52
+            # windows of 5 shared doublings; helios_scalarmult_fixed_precompute / helios_msm_fixed)
+            os.environ["HS_GPU_FIXED_WIDTH"] = "5"
+            try:
+                t0 = time.perf_counter()
+                f5 = hs.FixedSet(c, d_points=pts.data_ptr(), n=m + 1, stream=sh)
+                torch.cuda.synchronize()
+                build5 = time.perf_counter() - t0
+            finally:
+                del os.environ["HS_GPU_FIXED_WIDTH"]
+            out = torch.zeros((count, 8), dtype=torch.int32, device=dev)
+            f5.msm_device(ds.data_ptr(), count, out.data_ptr(), None, 0, hs.FIXED_COMB, sh)
+            torch.cuda.synchronize()
+            t = []
+            for _ in range(args.pedersen_steps):
+                t0 = time.perf_counter()
+                f5.msm_device(ds.data_ptr(), count, out.data_ptr(), None, 0, hs.FIXED_COMB, sh)
+                torch.cuda.synchronize()
+                t.append(time.perf_counter() - t0)
+            tm = float(np.median(t))
+            cv = ['helios', 'selene'][c]
+            res[f"{cv}_w5_table_build_ms"] = build5 * 1e3
+            res[f"{cv}_w5_ms"] = tm * 1e3
+            res[f"{cv}_w5_points_per_s"] = count * (m + 1) / tm
+            res[f"{cv}_w5_points_per_s_incl_build"] = count * (m + 1) / (tm + build5)
+            outs[(c, "w5")] = out.cpu().numpy()
+            f5.close()
+            res[f"{cv}_methods_agree"] = bool((outs[(c, "comb")] == outs[(c, "pippenger")]).all() and
+                                              (outs[(c, "comb")] == outs[(c, "w5")]).all())
             del ds
     best = sum(min(res[f"{cv}_comb_ms"], res[f"{cv}_pippenger_ms"]) for cv in ("helios", "selene"))
     res["points_per_s_best_method"] = 2 * count * (m + 1) / (best * 1e-3)
+    w5 = sum(res[f"{cv}_w5_ms"] for cv in ("helios", "selene"))
+    res["points_per_s_w5_tables"] = 2 * count * (m + 1) / (w5 * 1e-3)
+    res["note"] = ("comb: 16-tooth signed comb tables (8 GiB per set, built once in *_table_build_ms); w5: the "
+                   "width-5 affine tables the workload names (1 KiB per generator, built in *_w5_table_build_ms)")
     return res
 
 
@@ -345,31 +560,32 @@ def run_serving(hs, torch, args):
     }
 
 
-def cpu_baseline(hs_pts_cpu, sc_cpu, n_sample):
+def cpu_baseline(hs_pts_cpu, sc_cpu, n_sample, runs=3):
     """The reference CPU MSM (x64 Pippenger behind its runtime dispatch) on the first n_sample
-    inputs of each curve, one thread."""
-    lib = os.path.join(ROOT, "oracle", "_ref", "libhsref.so")
-    if not os.path.exists(lib):
+    inputs of each curve: one thread pinned to one core (as include/helioselene_benchmark.h:119-128
+    pins its benchmark thread), median of `runs` runs per curve."""
+    L = ref_lib()
+    if L is None:
         return None
-    L = ctypes.CDLL(lib)
-    L.hsref_init.restype = ctypes.c_uint
     feats = L.hsref_init()
-    tot = 0.0
-    for curve in (0, 1):
-        f = getattr(L, ("hsref_helios_time_msm", "hsref_selene_time_msm")[curve])
-        f.restype = ctypes.c_double
-        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
-        s = sc_cpu[curve][:n_sample].copy()
-        p = hs_pts_cpu[curve][:n_sample].copy()
-        tot += f(s.ctypes.data, p.ctypes.data, n_sample, 1, 1)
+    cpus = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {cpus[len(cpus) // 2]})
+    try:
+        med = [cpu_time_median(L, c, sc_cpu[c][:n_sample], hs_pts_cpu[c][:n_sample], runs) for c in (0, 1)]
+    finally:
+        os.sched_setaffinity(0, set(cpus))
+    tot = med[0] + med[1]
     return {
         "value": 2 * n_sample / tot,
         "unit": "points/s",
         "cores": 1,
         "kind": "reference",
+        "cpu_model": cpu_model(),
+        "per_curve_points_per_s": {"helios": n_sample / med[0], "selene": n_sample / med[1]},
         "sample": (f"one Helios and one Selene reference MSM (helios_msm_vartime / selene_msm_vartime via "
                    f"helioselene_init dispatch, cpu feature mask {feats:#x}) at n={n_sample} on the first "
-                   f"{n_sample} bases/scalars of this workload, 1 thread, {tot:.1f} s"),
+                   f"{n_sample} bases/scalars of this workload, 1 thread pinned to one core, median of {runs} "
+                   f"runs per curve ({med[0]:.2f} s + {med[1]:.2f} s)"),
     }
 
 
@@ -443,6 +659,9 @@ def main():
     ap.add_argument("--serve-bytes", type=int, default=16, help="server device budget, GiB")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=3)
+    ap.add_argument("--single-steps", type=int, default=10, help="timed runs per curve of the one-at-a-time legs")
+    ap.add_argument("--no-fcmpp", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in plan parameters)")
     ap.add_argument("--streams", type=int, default=4, choices=(1, 2, 4),
                     help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams; "
@@ -525,6 +744,20 @@ def main():
                 torch.cuda.synchronize()
         phases = [hs.phase_summary(c) for c in (0, 1)]
         hs.profile(False)
+        torch.cuda.synchronize()
+        # the timed steps' own results against the compiled reference's bytes on these inputs
+        ref = reference_results("plain")
+        rows = out.cpu().numpy()
+        parity = {}
+        for c, cv in enumerate(("helios", "selene")):
+            want = ref.get((cv, n, args.seed + 17 * c))
+            got = {rows[k].tobytes().hex() for k in range(4) if k % 2 == c and (args.streams == 4 or k < 2)}
+            parity[cv] = None if want is None else (got == {want})
+        parity["source"] = "tests/golden/workload_runs.json (compiled reference helios_msm_vartime / selene_msm_vartime)"
+        single = run_single(hs, torch, inputs, n, args, streams[0])
+        for c, cv in enumerate(("helios", "selene")):
+            want = ref.get((cv, n, args.seed + 17 * c))
+            single[cv]["bit_exact_vs_reference"] = None if want is None else single[cv].pop("result") == want
 
     elapsed = t1 - t0
     value = args.steps * 2 * n / elapsed
@@ -538,6 +771,24 @@ def main():
     msm_ms = [sum(per_curve_ms[cv].values()) for cv in ("helios", "selene")]
     msm_ops = [msm_int32_ops(c, n) for c in (0, 1)]
     traffic = measured_traffic(n)
+    cus, clk_khz, dev_name = hs.device_info()
+    nominal = 2.0 * cus * 4 * 16 * clk_khz * 1e3  # INT32 ops/s: one v_mad_u64_u32 per lane per clock
+    # HBM phases (fixed convention, c* of the INT32 formula): sort = scalars read once plus an 8-byte
+    # (key, index) record written and read per point and window; gather = one 64-byte affine base per
+    # point and window, during the bucket phase
+    W_star = -(-256 // cstar[0])
+    bytes_sort, bytes_gather = 32 * n + 16 * W_star * n, 64 * W_star * n
+    hbm = {}
+    for c, cv in enumerate(("helios", "selene")):
+        ph = per_curve_ms[cv]
+        t_sort = (ph.get("recode", 0.0) + ph.get("sort", 0.0)) * 1e-3
+        t_bucket = ph.get("bucket", 0.0) * 1e-3
+        hbm[cv] = {"sort_GBps": bytes_sort / t_sort / 1e9 if t_sort else None,
+                   "sort_frac": bytes_sort / t_sort / 8e12 if t_sort else None,
+                   "gather_GBps": bytes_gather / t_bucket / 1e9 if t_bucket else None,
+                   "gather_frac": bytes_gather / t_bucket / 8e12 if t_bucket else None}
+    hbm.update({"bytes_sort": bytes_sort, "bytes_gather": bytes_gather, "peak_GBps": 8000,
+                "phases": "sort = recode + sort phases, gather = bucket phase (isolated MSMs, HIP events)"})
 
     rec = {
         "metric": METRIC,
@@ -577,13 +828,32 @@ def main():
                                 f"{args.roofline_steps} isolated MSMs per curve after the timed steps",
             "peak_source": "measured v_mad_u64_u32 throughput x 2 INT32 ops per partial product",
             "whole_msm": {cv: {"ms": msm_ms[c], "int32_ops": msm_ops[c],
-                               "frac": msm_ops[c] / (msm_ms[c] * 1e-3) / peak}
+                               "frac": msm_ops[c] / (msm_ms[c] * 1e-3) / peak,
+                               "frac_of_nominal": msm_ops[c] / (msm_ms[c] * 1e-3) / nominal}
                           for c, cv in enumerate(("helios", "selene"))},
+            "peak_nominal": nominal / 1e12,
+            "peak_nominal_source": f"{cus} CUs x 4 SIMDs x 16 lanes x {clk_khz / 1e6:.2f} GHz x 2 ops per "
+                                   f"v_mad_u64_u32 ({dev_name})",
+            "frac_of_nominal": achieved / nominal,
+            "hbm_phases": hbm,
         },
+        "gate": {
+            "single_msm_frac_of_int32_peak": {cv: msm_ops[c] / (msm_ms[c] * 1e-3) / peak
+                                              for c, cv in enumerate(("helios", "selene"))},
+            "stream_frac_of_int32_peak": (msm_ops[0] + msm_ops[1]) * args.steps / elapsed / peak,
+            "target": 0.5,
+            "definition": "convention INT32 ops of one n=2^20 MSM / its isolated wall time (sum of its phases) / "
+                          "measured peak; stream_frac: the same over the timed steps (MSMs overlapping)",
+        },
+        "parity_vs_reference": parity,
+        "single_msm": single,
         "autotune": tuning,
         "phase_ms_isolated": per_curve_ms,
         "per_curve_points_per_s_isolated": {cv: n / (msm_ms[c] * 1e-3) for c, cv in enumerate(("helios", "selene"))},
     }
+    L = None if args.no_cpu else ref_lib()
+    rec["config1_helios_4096"] = None if args.no_config1 else run_config1(hs, torch, args, streams[0], L)
+    rec["fcmpp_verify"] = None if args.no_fcmpp else run_fcmpp(hs, torch, args, streams[0], L)
     rec["batched_mix"] = None if args.no_mix else run_mix(hs, torch, args)
     rec["pedersen_batch"] = None if args.no_pedersen else run_pedersen(hs, torch, args)
     rec["serving"] = None if args.no_serve else run_serving(hs, torch, args)

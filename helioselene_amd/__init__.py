@@ -105,6 +105,7 @@ def lib():
         L.hs_gpu_set_cpu_fallback.argtypes = [ctypes.c_int, CPU_MSM_FN, CPU_MSM_FIXED_FN]
         L.hs_gpu_fallback_count.restype = ctypes.c_uint64
         L.hs_gpu_set_device_limit.argtypes = [sz]
+        L.hs_gpu_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), cp, sz]
         L.hs_gpu_device_bytes.restype = sz
         _lib = L
     return _lib
@@ -294,6 +295,14 @@ def probe_int_mul_peak():
     v = ctypes.c_double()
     _check(lib().hs_gpu_probe_int_mul_peak(ctypes.byref(v)))
     return v.value
+
+
+def device_info():
+    """(compute units, peak clock kHz, device name) of the library's device."""
+    cu, clk = ctypes.c_int(), ctypes.c_int()
+    name = ctypes.create_string_buffer(256)
+    _check(lib().hs_gpu_device_info(ctypes.byref(cu), ctypes.byref(clk), name, 256))
+    return cu.value, clk.value, name.value.decode()
 
 
 def debug_dump(curve, which, max_items):

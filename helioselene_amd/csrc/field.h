@@ -730,8 +730,9 @@ template <class F> HS_HD void fe_pow(fe &r, const fe &a, const uint32_t e[8])
     r = acc;
 }
 
-// a^-1 = a^(m-2) (Fermat); maps 0 to 0 like the reference's fp_invert / fq_invert.
-template <class F> HS_HD void fe_inv(fe &r, const fe &a)
+// a^-1 = a^(m-2) (Fermat); maps 0 to 0 like the reference's fp_invert / fq_invert.  The kernels use
+// fe_inv (divsteps, inverse.h); this form checks it in the host tests.
+template <class F> HS_HD void fe_inv_fermat(fe &r, const fe &a)
 {
     uint32_t e[8];
 #pragma unroll
@@ -795,3 +796,5 @@ template <class F> HS_HD void fe_to_r51(uint64_t l[5], const fe &a)
 }
 
 } // namespace hs
+
+#include "inverse.h"

@@ -2396,6 +2396,26 @@ int hs_gpu_probe_int_mul_peak(double *products_per_second)
     return 0;
 }
 
+int hs_gpu_device_info(int *compute_units, int *max_clock_khz, char *name, size_t name_cap)
+{
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init()))
+        return r;
+    hipDeviceProp_t prop;
+    HS_CHECK(hipGetDeviceProperties(&prop, cx.device));
+    if (compute_units)
+        *compute_units = prop.multiProcessorCount;
+    if (max_clock_khz)
+        *max_clock_khz = prop.clockRate;
+    if (name && name_cap)
+    {
+        std::snprintf(name, name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    return 0;
+}
+
 int hs_gpu_map_to_curve(int curve, const unsigned char *u, size_t n, hs_gpu_jacobian *out_points, unsigned char *out32)
 {
     if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)

@@ -102,9 +102,13 @@ __device__ __forceinline__ void load_xyzz(xyzz &p, const xyzz *src)
 // ---------------------------------------------------------------- ingest
 // Reference points: helios_jacobian = {X[5], Y[5], Z[5]} radix-2^51 uint64 limbs, Z = 0 is the
 // identity (helios/include/helios_ops.h:44-64).  Output canonical affine (X/Z^2, Y/Z^3).
+// At most two waves per SIMD (the register allocation is padded to 169 VGPRs): with several MSMs
+// in flight, a full-occupancy ingest (107 VGPRs, four waves) crowds the CUs the other MSMs' bucket
+// kernels run on; four-stream throughput measured 650 M points/s against 720-730 M with the cap,
+// on the same box, with identical kernel code.
 template <class C>
-__global__ void __launch_bounds__(256) k_ingest(const uint64_t *__restrict__ jac, uint32_t n, uint32_t *__restrict__ out,
-                                                uint8_t *__restrict__ is_id)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_ingest(const uint64_t *__restrict__ jac, uint32_t n, uint32_t *__restrict__ out, uint8_t *__restrict__ is_id)
 {
     using F = typename C::F;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -726,7 +730,9 @@ __global__ void __launch_bounds__(64) k_horner(const jac *__restrict__ win_out, 
         acc = *total;
     for (; w >= w_lo; w--)
     {
-        jac_dbl_n_q<F>(acc, c);
+        // doublings of the identity are skipped: windows above every scalar's top digit are empty
+        if (!fe_is_zero_norm<F>(acc.Z))
+            jac_dbl_n_q<F>(acc, c);
         t = win_out[w];
         jac_add_q<F>(acc, acc, t);
     }
@@ -1116,7 +1122,8 @@ __global__ void __launch_bounds__(64) k_horner_batch(const jac *__restrict__ win
     acc = ws[W - 1];
     for (int w = W - 2; w >= 0; w--)
     {
-        jac_dbl_n_q<F>(acc, c);
+        if (!fe_is_zero_norm<F>(acc.Z)) // the identity doubles to itself
+            jac_dbl_n_q<F>(acc, c);
         t = ws[w];
         jac_add_q<F>(acc, acc, t);
     }

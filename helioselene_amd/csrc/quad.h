@@ -20,16 +20,40 @@ template <int K> __device__ __forceinline__ void fe_qbcast(fe &r, const fe &a)
         r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)a.v[i], K * 0x55, 0xf, 0xf, false);
 }
 
-// lane q of the quad takes a_q (masks keep the operands in registers: a select between the
-// structs themselves would be lowered to a select of their stack addresses)
+// lane q of the quad takes a_q.  Per-limb selects on the lane's two quad-index bits (v_cndmask with
+// loop-invariant lane masks: three per limb for four sources, two for three, one for two); a
+// select between the structs themselves would be lowered to a select of their stack addresses.
 __device__ __forceinline__ void fe_qsel(fe &r, const fe &a0, const fe &a1, const fe &a2, const fe &a3)
 {
-    const uint32_t q = threadIdx.x & 3u;
-    const uint32_t m0 = 0u - (uint32_t)(q == 0), m1 = 0u - (uint32_t)(q == 1), m2 = 0u - (uint32_t)(q == 2),
-                   m3 = 0u - (uint32_t)(q == 3);
+    const bool lo = (threadIdx.x & 1u) != 0, hi = (threadIdx.x & 2u) != 0;
 #pragma unroll
     for (int i = 0; i < NL; i++)
-        r.v[i] = (a0.v[i] & m0) | (a1.v[i] & m1) | (a2.v[i] & m2) | (a3.v[i] & m3);
+    {
+        const uint32_t x = lo ? a1.v[i] : a0.v[i];
+        const uint32_t y = lo ? a3.v[i] : a2.v[i];
+        r.v[i] = hi ? y : x;
+    }
+}
+
+// lanes 0, 1, 2, 3 take a0, a1, a2, a2
+__device__ __forceinline__ void fe_qsel3(fe &r, const fe &a0, const fe &a1, const fe &a2)
+{
+    const bool lo = (threadIdx.x & 1u) != 0, hi = (threadIdx.x & 2u) != 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        const uint32_t x = lo ? a1.v[i] : a0.v[i];
+        r.v[i] = hi ? a2.v[i] : x;
+    }
+}
+
+// lanes 0, 1, 2, 3 take a0, a1, a1, a1
+__device__ __forceinline__ void fe_qsel2(fe &r, const fe &a0, const fe &a1)
+{
+    const bool z = (threadIdx.x & 3u) == 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r.v[i] = z ? a0.v[i] : a1.v[i];
 }
 
 // four independent products a_k * b_k, one per lane of the quad
@@ -52,8 +76,8 @@ __device__ __forceinline__ void fe_mul_q3(fe &r0, fe &r1, fe &r2, const fe &a0, 
                                           const fe &b1, const fe &a2, const fe &b2)
 {
     fe a, b, r;
-    fe_qsel(a, a0, a1, a2, a2);
-    fe_qsel(b, b0, b1, b2, b2);
+    fe_qsel3(a, a0, a1, a2);
+    fe_qsel3(b, b0, b1, b2);
     fe_mul<F>(r, a, b);
     fe_qbcast<0>(r0, r);
     fe_qbcast<1>(r1, r);
@@ -65,7 +89,7 @@ template <class F>
 __device__ __forceinline__ void fe_sq_q3(fe &r0, fe &r1, fe &r2, const fe &a0, const fe &a1, const fe &a2)
 {
     fe a, r;
-    fe_qsel(a, a0, a1, a2, a2);
+    fe_qsel3(a, a0, a1, a2);
     fe_sq<F>(r, a);
     fe_qbcast<0>(r0, r);
     fe_qbcast<1>(r1, r);
@@ -76,8 +100,8 @@ template <class F>
 __device__ __forceinline__ void fe_mul_q2(fe &r0, fe &r1, const fe &a0, const fe &b0, const fe &a1, const fe &b1)
 {
     fe a, b, r;
-    fe_qsel(a, a0, a1, a1, a1);
-    fe_qsel(b, b0, b1, b1, b1);
+    fe_qsel2(a, a0, a1);
+    fe_qsel2(b, b0, b1);
     fe_mul<F>(r, a, b);
     fe_qbcast<0>(r0, r);
     fe_qbcast<1>(r1, r);

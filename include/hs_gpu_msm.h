@@ -282,6 +282,9 @@ int hs_gpu_phase_times(float *ms, int cap);
 int hs_gpu_phase_summary(int curve, float *ms, int cap, int *calls);
 const char *hs_gpu_phase_name(int i);
 int hs_gpu_probe_int_mul_peak(double *products_per_second);
+/* Compute units, peak engine clock (kHz) and name of the library's device: the nominal issue
+ * rate of the multiply is compute_units x 4 SIMDs x 16 lanes x clock. */
+int hs_gpu_device_info(int *compute_units, int *max_clock_khz, char *name, size_t name_cap);
 /* Copies intermediate state of the last plain MSM on this device (inspection in tests): which = 0
  * window sums, 1 segment sums, 2 bucket sums (as 32-byte encodings), 3 sorted bucket entries,
  * 4 bucket offsets (uint32).  *count receives the number of items written. */

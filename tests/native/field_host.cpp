@@ -54,6 +54,7 @@ template <class F> static void op(int o, uint8_t *out, const uint8_t *pa, const 
     case 5: fe_canon<F>(r, a); break;
     case 6: fe_neg<F>(r, a); break;
     case 7: fe_mul_small<F>(r, a, pb[0] | (pb[1] << 8)); break;
+    case 8: fe_inv_fermat<F>(r, a); break;
     default: fe_zero(r);
     }
     st(out, r);

@@ -47,6 +47,25 @@ def test_field_ops_vs_bigint(L, k):
 
 
 @pytest.mark.parametrize("k", [0, 1])
+def test_divstep_inverse(L, k):
+    """fe_inv (divsteps) against Python's modular inverse and the Fermat form: 0, small values,
+    m - 1, non-canonical inputs (>= m, up to 2^256 - 1), powers of two, values just below m, and
+    random values; canonical outputs."""
+    m = O.FIELD[k]
+    rnd = random.Random(77 + k)
+    vals = [0, 1, 2, 3, m - 1, m - 2, m, m + 1, 2 * m - 1, (1 << 256) - 1, (1 << 255) - 1, (1 << 254),
+            (m + 1) // 2, m // 3]
+    vals += [1 << i for i in range(0, 256, 7)] + [m - (1 << i) for i in range(0, 250, 9)]
+    vals += [rnd.getrandbits(256) for _ in range(3000)] + [rnd.randrange(m) for _ in range(3000)]
+    for a in vals:
+        r = call(L, k, 4, a)
+        want = 0 if a % m == 0 else pow(a, -1, m)
+        assert r == want, hex(a)
+    for a in vals[:200]:
+        assert call(L, k, 8, a) == call(L, k, 4, a)
+
+
+@pytest.mark.parametrize("k", [0, 1])
 def test_r51_conversion(L, k):
     """Reference radix-2^51 limbs (lazy: limbs up to 64 bits) in and canonical limbs out."""
     m = O.FIELD[k]
