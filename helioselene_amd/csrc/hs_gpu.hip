@@ -313,6 +313,15 @@ struct Workspace
     }
 };
 
+// Scratch of the no-allocation entry point (hs_gpu_msm_device_noalloc): one array per device, in
+// the code object, so the entry point itself allocates nothing.  Host code reaches it only through
+// the device address hipGetSymbolAddress returns at initialisation (Ctx::scratch_ptr); the array's
+// name is a host-side stand-in.  Every call on a device shares it: calls that overlap on one device
+// race.  64 MiB hold the plan of an MSM of up to 2^16 points.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr size_t NOALLOC_MAX_POINTS = (size_t)1 << 16;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 // Staging of one host-pointer MSM in flight: a stream of its own (and with it a workspace of its
 // own) plus the upload and result buffers.  Concurrent host callers each take one from a pool.
 struct HostStage
@@ -328,6 +337,8 @@ struct Ctx
     std::mutex mu;
     DevBuf scalars, points, out32, outjac, status, nbad, probe;
     std::vector<HostStage *> stages_free, stages_all;
+    unsigned char *scratch_ptr = nullptr; // device address of `scratch` on this context's device
+    Workspace noalloc_ws;                 // buffers carved from it (never released)
     std::vector<std::pair<hipStream_t, Workspace *>> wss;
     bool ready = false;
 
@@ -394,6 +405,7 @@ struct Ctx
             return fail(HS_GPU_EDEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
         device = dev;
         HS_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HS_CHECK(hipGetSymbolAddress((void **)&scratch_ptr, HIP_SYMBOL(scratch)));
         ready = true;
         return 0;
     }
@@ -453,41 +465,80 @@ static uint32_t reduce_win_per_block(uint32_t segs_per_win)
 // Point indices in the sort entries have 24 bits: larger MSMs run as chunks (msm_device_any).
 constexpr size_t MSM_MAX_CHUNK = (size_t)1 << 23;
 
+// Launch geometry of a plain MSM: two-level sort with nsb super-buckets of 2^shift buckets per
+// window and coarse blocks of T points (about 256 blocks, one per CU, at least 256 points each).
+struct MsmGeom
+{
+    int shift, nsb;
+    uint32_t T, nblk, nh, scan_len;
+    size_t lds, max_chunks;
+};
+
+MsmGeom msm_geom(const Plan &P, size_t n)
+{
+    MsmGeom g;
+    const uint32_t nn = (uint32_t)n;
+    g.shift = P.c - 1 > 8 ? P.c - 1 - 8 : 0;
+    g.nsb = (int)(P.B >> g.shift);
+    g.T = 256;
+    while ((nn + g.T - 1) / g.T > 256)
+        g.T *= 2;
+    g.nblk = (nn + g.T - 1) / g.T;
+    g.nh = (uint32_t)P.W * g.nsb;
+    g.lds = (size_t)g.nh * 4;
+    g.scan_len = g.nh * g.nblk + 1;
+    g.max_chunks = (n * (size_t)P.W + P.L - 1) / P.L; // bound for any window group
+    return g;
+}
+
+// Every device buffer one plain MSM of n points uses, with its size (one list for the workspace
+// reservations and for carving the no-allocation scratch).
+std::vector<std::pair<DevBuf *, size_t>> msm_buffers(Workspace &ws, const Plan &P, const MsmGeom &g, size_t n)
+{
+    const uint32_t per = hs::SCAN_BLOCK * hs::SCAN_ITEMS, sblk = (g.scan_len + per - 1) / per;
+    return {{&ws.aff, n * 64},
+            {&ws.isid, n},
+            {&ws.digits, n * 2 * (size_t)P.W},
+            {&ws.offsets, (size_t)(P.nb + 1) * 4},
+            {&ws.entries, n * 4 * (size_t)P.W + 4},
+            {&ws.cnt_a, (size_t)g.scan_len * 4},
+            {&ws.off_a, (size_t)g.scan_len * 4},
+            {&ws.sorted, n * 4 * (size_t)P.W + 4},
+            {&ws.buckets, (size_t)P.nb * sizeof(hs::xyzz)},
+            {&ws.seg, (size_t)P.segs * sizeof(hs::xyzz)},
+            {&ws.win, (size_t)P.W * sizeof(hs::xyzz)},
+            {&ws.big_list, (size_t)P.nb * 4},
+            {&ws.big_count, 64},
+            {&ws.total, sizeof(hs::jac)},
+            {&ws.chunk_bucket, g.max_chunks * 4 + 4},
+            {&ws.bnd, 2 * g.max_chunks * sizeof(hs::xyzz)},
+            {&ws.block_sums, 2 * (size_t)(sblk + 1) * 4 + 64}};
+}
+
 // compat_n: the MSM size that decides the reference's window-8 behaviour (the whole MSM's size
-// when this call is one chunk of it).
+// when this call is one chunk of it).  With `fixed_ws` (the no-allocation entry point) the plan is
+// `*fixed_plan`, every buffer is already carved from the scratch array, and nothing that allocates
+// runs: no profiling events, no auxiliary stream (one window group).
 template <class C>
 int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, size_t n, unsigned flags, uint32_t *d_out32,
-               uint64_t *d_outjac, hipStream_t st, size_t compat_n = 0)
+               uint64_t *d_outjac, hipStream_t st, size_t compat_n = 0, Workspace *fixed_ws = nullptr,
+               const Plan *fixed_plan = nullptr)
 {
     if (n >= (1ull << 24))
         return fail(HS_GPU_EINVAL, "n must be below 2^24 points per chunk");
-    Workspace &ws = cx.ws(st);
-    const Plan P = make_plan(C::id, n);
+    Workspace &ws = fixed_ws ? *fixed_ws : cx.ws(st);
+    const Plan P = fixed_plan ? *fixed_plan : make_plan(C::id, n);
     const uint32_t nn = (uint32_t)n;
     const bool compat = !(flags & HS_GPU_EXACT) && ref_window(compat_n ? compat_n : n) == 8;
-    // two-level sort geometry: nsb super-buckets of 2^shift buckets per window, blocks of T points
-    const int shift = P.c - 1 > 8 ? P.c - 1 - 8 : 0;
-    const int nsb = (int)(P.B >> shift);
-    // points per coarse block: about 256 blocks (one per CU), at least 256 points each
-    uint32_t T = 256;
-    while ((nn + T - 1) / T > 256)
-        T *= 2;
-    const uint32_t nblk = (nn + T - 1) / T;
-    const uint32_t nh = (uint32_t)P.W * nsb;
-    const size_t lds = (size_t)nh * 4;
+    const MsmGeom geo = msm_geom(P, n);
+    const int shift = geo.shift, nsb = geo.nsb;
+    const uint32_t T = geo.T, nblk = geo.nblk, nh = geo.nh;
+    const size_t lds = geo.lds;
     int r;
-    if ((r = ws.aff.reserve(n * 64)) || (r = ws.isid.reserve(n)) || (r = ws.digits.reserve(n * 2 * (size_t)P.W)) ||
-        (r = ws.offsets.reserve((size_t)(P.nb + 1) * 4)) || (r = ws.entries.reserve(n * 4 * (size_t)P.W + 4)) ||
-        (r = ws.cnt_a.reserve(((size_t)nh * nblk + 1) * 4)) || (r = ws.off_a.reserve(((size_t)nh * nblk + 1) * 4)) ||
-        (r = ws.sorted.reserve(n * 4 * (size_t)P.W + 4)) || (r = ws.buckets.reserve((size_t)P.nb * sizeof(hs::xyzz))) ||
-        (r = ws.seg.reserve((size_t)P.segs * sizeof(hs::xyzz))) || (r = ws.win.reserve((size_t)P.W * sizeof(hs::xyzz))) ||
-        (r = ws.big_list.reserve((size_t)P.nb * 4)) || (r = ws.big_count.reserve(64)) ||
-        (r = ws.total.reserve(sizeof(hs::jac))))
-        return r;
+    for (const auto &b : msm_buffers(ws, P, geo, n))
+        if ((r = b.first->reserve(b.second)))
+            return r;
     const uint32_t L = P.L;
-    const size_t max_chunks = (n * (size_t)P.W + L - 1) / L; // bound for any window group
-    if ((r = ws.chunk_bucket.reserve(max_chunks * 4 + 4)) || (r = ws.bnd.reserve(2 * max_chunks * sizeof(hs::xyzz))))
-        return r;
     uint32_t *aff = ws.aff.as<uint32_t>();
     uint8_t *isid = ws.isid.as<uint8_t>();
     uint16_t *digits = ws.digits.as<uint16_t>();
@@ -500,11 +551,13 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     const dim3 blk(256), grd((nn + 255) / 256);
     // Window groups (Q > 1) pipeline the combine on an auxiliary stream; with one group the whole
     // MSM stays on st (no cross-stream hop for the caller's stream to wait on).
-    const int Q = P.Q < Workspace::MAXQ ? P.Q : Workspace::MAXQ;
+    const int Q = fixed_ws ? 1 : P.Q < Workspace::MAXQ ? P.Q : Workspace::MAXQ;
     if (Q > 1 && (r = ws.ensure_aux()))
         return r;
     hipStream_t cst = Q > 1 ? ws.aux : st;
-    if ((r = cx.begin_profile(C::id)))
+    if (fixed_ws)
+        cx.cur = nullptr; // no profiling events on the no-allocation path
+    else if ((r = cx.begin_profile(C::id)))
         return r;
     cx.mark(st, -1);
     hipLaunchKernelGGL(hs::k_ingest<C>, grd, blk, 0, st, d_points, nn, aff, isid);
@@ -513,7 +566,7 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
                        P.c, P.W, compat ? 1 : 0, nsb, shift, T, nblk, digits, cnt_a);
     cx.mark(st, 1);
     hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, cnt_a + (size_t)nh * nblk, 1u, 0u);
-    if ((r = scan_exclusive(ws, cnt_a, off_a, nh * nblk + 1, st)))
+    if ((r = scan_exclusive(ws, cnt_a, off_a, geo.scan_len, st)))
         return r;
     hipLaunchKernelGGL(hs::k_coarse_scatter, dim3(nblk), dim3(hs::SORT_COARSE_THREADS), lds, st, digits, nn, P.W, nsb, shift, T, nblk, off_a,
                        entries);
@@ -2011,6 +2064,61 @@ int hs_gpu_msm_device(int curve, const void *d_scalars, const void *d_points, si
         return r;
     hipStream_t st = stream ? (hipStream_t)stream : cx.stream;
     return msm_device_any(curve, cx, d_scalars, d_points, n, flags, d_out32, d_out_jac, st);
+}
+
+// Plan of the no-allocation entry point: the built-in parameters (window bits choose_c(n), run
+// length 32, segments of 8 buckets, one window group), independent of the tuning table and the
+// environment, so the scratch it needs is a function of n alone.
+static Plan noalloc_plan(size_t n)
+{
+    Plan p;
+    p.c = choose_c(n);
+    p.W = (256 + p.c - 1) / p.c;
+    p.B = 1u << (p.c - 1);
+    p.G = p.B < 8u ? p.B : 8u;
+    p.nb = (uint32_t)p.W * p.B;
+    p.L = 32;
+    p.Q = 1;
+    p.segs = p.nb / p.G;
+    return p;
+}
+
+int hs_gpu_msm_device_noalloc(int curve, const void *d_scalars, const void *d_points, size_t n, unsigned flags,
+                              void *d_out32, void *d_out_jac, void *stream)
+{
+    if (curve != HS_GPU_HELIOS && curve != HS_GPU_SELENE)
+        return fail(HS_GPU_EINVAL, "unknown curve id");
+    if (n == 0 || !d_scalars || !d_points || !d_out32)
+        return fail(HS_GPU_EINVAL, "hs_gpu_msm_device_noalloc needs n > 0 and device buffers");
+    if (n > NOALLOC_MAX_POINTS)
+        return fail(HS_GPU_EINVAL, "hs_gpu_msm_device_noalloc takes at most " + std::to_string(NOALLOC_MAX_POINTS) +
+                                       " points (its scratch holds " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                       " bytes); n = " + std::to_string(n));
+    Ctx &cx = ctx();
+    std::lock_guard<std::mutex> lk(cx.mu);
+    int r;
+    if ((r = cx.init())) // first call only; hs_gpu_init beforehand keeps every call allocation-free
+        return r;
+    const Plan P = noalloc_plan(n);
+    const MsmGeom geo = msm_geom(P, n);
+    // carve every buffer of the plan from the scratch array (256-byte aligned)
+    size_t off = 0;
+    for (const auto &b : msm_buffers(cx.noalloc_ws, P, geo, n))
+    {
+        off = (off + 255) & ~(size_t)255;
+        if (off + b.second > NOALLOC_SCRATCH_BYTES)
+            return fail(HS_GPU_EINVAL, "hs_gpu_msm_device_noalloc: the plan of n = " + std::to_string(n) +
+                                           " does not fit the scratch array");
+        b.first->p = cx.scratch_ptr + off;
+        b.first->cap = b.second;
+        off += b.second;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : cx.stream;
+    return curve == HS_GPU_HELIOS
+               ? msm_device<hs::HeliosC>(cx, (const uint32_t *)d_scalars, (const uint64_t *)d_points, n, flags,
+                                         (uint32_t *)d_out32, (uint64_t *)d_out_jac, st, 0, &cx.noalloc_ws, &P)
+               : msm_device<hs::SeleneC>(cx, (const uint32_t *)d_scalars, (const uint64_t *)d_points, n, flags,
+                                         (uint32_t *)d_out32, (uint64_t *)d_out_jac, st, 0, &cx.noalloc_ws, &P);
 }
 
 int hs_gpu_synchronize(void)

@@ -102,6 +102,17 @@ int hs_gpu_msm_ex(int curve, hs_gpu_jacobian *result, unsigned char out32[32], c
 int hs_gpu_msm_device(int curve, const void *d_scalars, const void *d_points, size_t n, unsigned flags,
                       void *d_out32, void *d_out_jac, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  hs_gpu_msm_device with every
+ * scratch buffer carved from one static 64 MiB device array per device: it calls no hipMalloc,
+ * hipFree or allocating library routine once the library is initialised (hs_gpu_init; otherwise the
+ * first call initialises it).  It takes n <= 65536 points (HS_GPU_EINVAL beyond), uses the built-in
+ * plan parameters (not the tuning table) and gives the same results as hs_gpu_msm_device.  The
+ * array keeps the last call's contents; every call on a device shares it, so calls that overlap on
+ * one device (on different streams, or from several threads without ordering) race. */
+int hs_gpu_msm_device_noalloc(int curve, const void *d_scalars, const void *d_points, size_t n, unsigned flags,
+                              void *d_out32, void *d_out_jac, void *stream);
+
 /* Waits for all work the library queued on its own stream(s). */
 int hs_gpu_synchronize(void);
 
