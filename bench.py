@@ -662,6 +662,9 @@ def main():
     ap.add_argument("--single-steps", type=int, default=10, help="timed runs per curve of the one-at-a-time legs")
     ap.add_argument("--no-fcmpp", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--affine-k", type=int, default=0, choices=(0, 1, 2, 4),
+                    help="plain MSMs with the batched-affine bucket kernel, K chains per lane (measurement of that "
+                         "kernel; 0 = the default xyzz kernel)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in plan parameters)")
     ap.add_argument("--streams", type=int, default=4, choices=(1, 2, 4),
                     help="1: both MSMs of a step on one stream; 2: Helios and Selene on their own streams; "
@@ -672,6 +675,8 @@ def main():
               f"MI355X; multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
         sys.exit(2)
 
+    if args.affine_k:
+        os.environ["HS_GPU_AFFINE_K"] = str(args.affine_k)
     import numpy as np
     import torch
 
@@ -810,6 +815,7 @@ def main():
             "curves": ["helios", "selene"],
             "window_bits": [hs.window_bits(c, n) for c in (0, 1)],
             "result_mode": "reference-compatible",
+            "bucket_method": f"batched affine, {args.affine_k} chains per lane" if args.affine_k else "xyzz mixed additions",
             "streams": args.streams,
         },
         "roofline": {

@@ -308,11 +308,11 @@ def device_info():
 def debug_dump(curve, which, max_items):
     """Intermediate state of the last plain MSM: which 0/1/2 -> list of 32-byte encodings of the
     window / segment / bucket sums; 3/4 -> uint32 array of sorted entries / bucket offsets."""
-    item = 32 if which < 3 else 4
+    item = 32 if which in (0, 1, 2, 5) else 4
     buf = np.zeros(max_items * item, dtype=np.uint8)
     cnt = ctypes.c_size_t()
     _check(lib().hs_gpu_debug_dump(CURVES[curve], which, _ptr(buf), buf.size, ctypes.byref(cnt)))
-    if which < 3:
+    if which in (0, 1, 2, 5):
         return [buf[32 * i:32 * i + 32].tobytes() for i in range(cnt.value)]
     return buf.view(np.uint32)[:cnt.value].copy()
 

@@ -20,6 +20,7 @@
 #include "../../include/hs_gpu_msm.h"
 #include "msm_kernels.h"
 #include "fixed_kernels.h"
+#include "affine_kernels.h"
 
 namespace
 {
@@ -139,6 +140,7 @@ struct Plan
 {
     int c, W, Q;
     uint32_t B, G, nb, segs, L;
+    int affK = 0; // bucket method: 0 = xyzz mixed additions (k_bucket_flat), K = batched affine with K chains per lane
 };
 
 // Window width for an n-point MSM: balances W*n bucket additions against the W*2^c reduction
@@ -260,6 +262,10 @@ Plan make_plan(int curve, size_t n)
     p.Q = env_int("HS_GPU_Q", 1);
     if (p.Q < 1)
         p.Q = 1;
+    // HS_GPU_AFFINE_K = 1, 2 or 4 selects the batched-affine bucket kernel (measurements and tests)
+    p.affK = env_int("HS_GPU_AFFINE_K", 0);
+    if (p.affK != 1 && p.affK != 2 && p.affK != 4)
+        p.affK = 0;
     p.segs = p.nb / p.G;
     return p;
 }
@@ -338,6 +344,7 @@ struct Ctx
     DevBuf scalars, points, out32, outjac, status, nbad, probe;
     std::vector<HostStage *> stages_free, stages_all;
     unsigned char *scratch_ptr = nullptr; // device address of `scratch` on this context's device
+    Workspace *last_ws = nullptr;         // workspace of the last plain MSM (hs_gpu_debug_dump)
     Workspace noalloc_ws;                 // buffers carved from it (never released)
     std::vector<std::pair<hipStream_t, Workspace *>> wss;
     bool ready = false;
@@ -527,6 +534,7 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     if (n >= (1ull << 24))
         return fail(HS_GPU_EINVAL, "n must be below 2^24 points per chunk");
     Workspace &ws = fixed_ws ? *fixed_ws : cx.ws(st);
+    cx.last_ws = &ws;
     const Plan P = fixed_plan ? *fixed_plan : make_plan(C::id, n);
     const uint32_t nn = (uint32_t)n;
     const bool compat = !(flags & HS_GPU_EXACT) && ref_window(compat_n ? compat_n : n) == 8;
@@ -587,8 +595,22 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         const size_t chunks = ((size_t)n * (w_hi - w_lo) + L - 1) / L;
         hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, big_count, 1u, 0u);
         hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>());
-        hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((chunks + 255) / 256)), blk, 0, st, offsets, sorted,
-                           g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+        if (P.affK == 0)
+            hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((chunks + 255) / 256)), blk, 0, st, offsets, sorted,
+                               g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+        else
+        {
+            const uint32_t per = 64u * (uint32_t)P.affK, grid = (uint32_t)((chunks + per - 1) / per);
+            if (P.affK == 1)
+                hipLaunchKernelGGL((hs::k_bucket_affine<C, 1>), dim3(grid), dim3(64), 0, st, offsets, sorted, g_lo, g_hi, L,
+                                   ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+            else if (P.affK == 2)
+                hipLaunchKernelGGL((hs::k_bucket_affine<C, 2>), dim3(grid), dim3(64), 0, st, offsets, sorted, g_lo, g_hi, L,
+                                   ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+            else
+                hipLaunchKernelGGL((hs::k_bucket_affine<C, 4>), dim3(grid), dim3(64), 0, st, offsets, sorted, g_lo, g_hi, L,
+                                   ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
+        }
         cx.mark(st, 3);
         hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.bnd.as<hs::xyzz>(), 8u,
                            buckets, big_list, big_count);
@@ -2429,7 +2451,9 @@ int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t 
     if (!cx.ready)
         return fail(HS_GPU_EINVAL, "no MSM has run");
     *count = 0;
-    Workspace &ws = cx.ws(cx.stream);
+    if (!cx.last_ws)
+        return fail(HS_GPU_EINVAL, "no MSM has run");
+    Workspace &ws = *cx.last_ws;
     if (which == 3 || which == 4)
     {
         // 3: sorted entries, 4: bucket offsets (raw uint32)
@@ -2439,9 +2463,12 @@ int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t 
         *count = nb / 4;
         return 0;
     }
-    DevBuf &src = which == 0 ? ws.win : which == 1 ? ws.seg : ws.buckets;
+    DevBuf &src = which == 0 ? ws.win : which == 1 ? ws.seg : which == 5 ? ws.bnd : ws.buckets;
     // window sums are Jacobian, segment and bucket sums xyzz
     size_t n = std::min(cap_bytes / 32, src.cap / (which == 0 ? sizeof(hs::jac) : sizeof(hs::xyzz)));
+    if (n == 0)
+        return 0;
+    HS_CHECK(hipDeviceSynchronize()); // the last MSM may have run on any stream
     int r;
     if ((r = cx.probe.reserve(n * 32 + 64)))
         return r;

@@ -240,3 +240,39 @@ def test_drop_in_slot_and_n0():
     assert GH.jac_to_bytes(0, jac) == O.scalar_mul(0, (15).to_bytes(32, "little"), g)
     out, jac = hs.msm(0, np.zeros((0, 32), np.uint8), np.zeros((0, 15), np.uint64))
     assert out == bytes(32) and GH.jac_to_bytes(0, jac) == bytes(32)
+
+
+@pytest.mark.parametrize("K", [1, 2, 4])
+@pytest.mark.parametrize("c", [0, 1])
+def test_affine_bucket_method(c, K, monkeypatch):
+    """The batched-affine bucket kernel (K chains per lane, one wavefront-wide batch inversion per
+    step), forced for the plain path: duplicated points (doubling in a bucket), P and -P in one
+    bucket (cancellation), identities, edge scalars, random Z; the window-8 band both ways; and the
+    2^16 plain workload against the compiled reference's bytes."""
+    import json
+    import os
+
+    from helioselene_amd import workloads as W
+
+    monkeypatch.setenv("HS_GPU_AFFINE_K", str(K))
+    g = O.generator(c)
+    order = O.ORDER[c]
+    n = 40
+    pts = GH.to_jac_array(c, g * n, n)
+    sc = (77).to_bytes(32, "little") * n
+    assert run(c, sc, pts)[0] == O.scalar_mul(c, (77 * n).to_bytes(32, "little"), g)
+    s = 123456789
+    sc = (s.to_bytes(32, "little") + (order - s).to_bytes(32, "little")) * (n // 2)
+    assert run(c, sc, pts)[0] == bytes(32)
+    for n in (3, 100, 700, 1500, 5000):
+        rnd = random.Random(300 * c + n)
+        sc = O.gen_scalars(c, 90 + n, n, edge_frac=0.1)
+        pb = O.gen_sswu_bases(c, 91 + n, n, dup_frac=0.1, id_frac=0.05)
+        pts = GH.to_jac_array(c, pb, n, rnd=rnd, lazy=True)
+        assert run(c, sc, pts)[0] == O.msm(c, sc, pb, n), n
+        assert run(c, sc, pts, flags=hs.EXACT)[0] == O.msm_naive(c, sc, pb, n), n
+    runs = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "workload_runs.json")))
+    r = [x for x in runs if x["kind"] == "plain" and x["curve"] == O.CURVES[c] and x["n"] == 1 << 16][0]
+    w = W.plain_inputs(c, r["n"], r["seed"])
+    pts = W.apply_dups_and_identities(hs.map_to_curve(c, w["u"].tobytes())[0], w)
+    assert hs.msm(c, w["scalars"], pts)[0].hex() == r["ref_result"]
