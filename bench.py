@@ -733,10 +733,12 @@ def main():
             step(i)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        # HBM counter calibration: one elementwise kernel that reads 256 MiB and writes 256 MiB, so
-        # a FETCH_SIZE / WRITE_SIZE counter run of this command can scale the MSM kernels' counts
-        cal = torch.zeros(1 << 26, dtype=torch.int32, device=dev)
+        # HBM counter calibration: one elementwise kernel that reads 2 GiB and writes 2 GiB (eight
+        # times the 256 MB last-level cache, so nothing it reads is cache-resident), so a FETCH_SIZE /
+        # WRITE_SIZE counter run of this command can scale the MSM kernels' counts
+        cal = torch.zeros(1 << 29, dtype=torch.int32, device=dev)
         cal2 = torch.empty_like(cal)
+        torch.cuda.synchronize()
         torch.add(cal, 1, out=cal2)
         torch.cuda.synchronize()
         del cal, cal2
