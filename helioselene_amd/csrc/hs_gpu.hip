@@ -166,6 +166,13 @@ int env_int(const char *name, int dflt)
     return v ? std::atoi(v) : dflt;
 }
 
+// Horner combines run with the row-distributed arithmetic (one wavefront per chain); HS_GPU_HORNER=quad
+// selects the earlier quad-of-lanes kernels (same results) for comparisons.
+const bool g_horner_quad = [] {
+    const char *v = std::getenv("HS_GPU_HORNER");
+    return v && std::strcmp(v, "quad") == 0;
+}();
+
 // ---------------------------------------------------------------- tuned plan parameters
 // The autotuner's table: for (mode, curve, size class) the fastest (c, L, G) it measured.  mode 0 =
 // plain MSM, 1 = a group of a batch; size class = floor(log2 n).  Lookups fall back to the nearest
@@ -629,7 +636,11 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
             HS_CHECK(hipEventRecord(ws.ev_group[q], st));
             HS_CHECK(hipStreamWaitEvent(cst, ws.ev_group[q], 0));
         }
-        hipLaunchKernelGGL(hs::k_horner<C>, dim3(1), dim3(64), 0, cst, win, w_lo, w_hi, P.c, q == 0 ? 1 : 0, total);
+        if (g_horner_quad)
+            hipLaunchKernelGGL(hs::k_horner<C>, dim3(1), dim3(64), 0, cst, win, w_lo, w_hi, P.c, q == 0 ? 1 : 0, total);
+        else
+            hipLaunchKernelGGL(hs::k_horner_row<C>, dim3(1), dim3(64), 0, cst, win, w_lo, w_hi, P.c, q == 0 ? 1 : 0,
+                               total);
     }
     hipLaunchKernelGGL(hs::k_finalize<C>, dim3(1), dim3(64), 0, cst, total, d_out32, d_outjac);
     if (flags & HS_GPU_ERASE)
@@ -731,8 +742,12 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
     const uint32_t wpb = reduce_win_per_block(g.B / g.G);
     hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((g.V + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, g.B / g.G, 0u,
                        g.V, win);
-    hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((4 * g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
-                       g.total.as<hs::jac>());
+    if (g_horner_quad)
+        hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((4 * g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
+                           g.total.as<hs::jac>());
+    else
+        hipLaunchKernelGGL(hs::k_horner_batch_row<C>, dim3(g.M), dim3(64), 0, st, win, g.M, g.W, g.c,
+                           g.total.as<hs::jac>());
     hipLaunchKernelGGL(hs::k_finalize_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, g.total.as<hs::jac>(), d, g.M);
     HS_CHECK(hipGetLastError());
     return 0;

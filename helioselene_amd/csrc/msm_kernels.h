@@ -16,6 +16,7 @@
 #include "curve.h"
 #include "scalar.h"
 #include "quad.h"
+#include "rowcurve.h"
 
 namespace hs
 {
@@ -740,6 +741,23 @@ __global__ void __launch_bounds__(64) k_horner(const jac *__restrict__ win_out, 
         *total = acc;
 }
 
+// The same Horner step with every field element spread over a DPP row (rowcurve.h): one whole
+// wavefront, four products per formula level.
+template <class C>
+__global__ void __launch_bounds__(64) k_horner_row(const jac *__restrict__ win_out, int w_lo, int w_hi, int c,
+                                                   int first, jac *total)
+{
+    using F = typename C::F;
+    row::RowK<F> k;
+    row::rowk_init(k);
+    row::rjac acc;
+    row::rhorner<F>(k, acc, win_out, w_lo, w_hi, c, first != 0, total);
+    jac r;
+    row::rjac_get<F>(r, acc);
+    if (threadIdx.x == 0)
+        *total = r;
+}
+
 // out32: compressed point (8 words); out_jac: reference Jacobian struct (15 uint64): (x : y : 1)
 // canonical, or (1 : 1 : 0) for the identity (helios_ops.h:44-50).
 template <class C>
@@ -1129,6 +1147,25 @@ __global__ void __launch_bounds__(64) k_horner_batch(const jac *__restrict__ win
     }
     if ((threadIdx.x & 3) == 0)
         total[m] = acc;
+}
+
+// Horner per MSM with the row-distributed arithmetic: one wavefront (workgroup) per MSM.
+template <class C>
+__global__ void __launch_bounds__(64) k_horner_batch_row(const jac *__restrict__ win_out, uint32_t M, int W, int c,
+                                                         jac *__restrict__ total)
+{
+    using F = typename C::F;
+    const uint32_t m = blockIdx.x;
+    if (m >= M)
+        return;
+    row::RowK<F> k;
+    row::rowk_init(k);
+    row::rjac acc;
+    row::rhorner<F>(k, acc, win_out + (size_t)m * W, 0, W, c, true, nullptr);
+    jac r;
+    row::rjac_get<F>(r, acc);
+    if (threadIdx.x == 0)
+        total[m] = r;
 }
 
 template <class C>
