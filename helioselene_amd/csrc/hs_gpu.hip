@@ -639,10 +639,15 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         if (g_horner_quad)
             hipLaunchKernelGGL(hs::k_horner<C>, dim3(1), dim3(64), 0, cst, win, w_lo, w_hi, P.c, q == 0 ? 1 : 0, total);
         else
+        {
+            // the last group's kernel also finalizes (w_lo == 0 only there)
+            const bool last = w_lo == 0;
             hipLaunchKernelGGL(hs::k_horner_row<C>, dim3(1), dim3(64), 0, cst, win, w_lo, w_hi, P.c, q == 0 ? 1 : 0,
-                               total);
+                               total, last ? d_out32 : nullptr, last ? d_outjac : nullptr);
+        }
     }
-    hipLaunchKernelGGL(hs::k_finalize<C>, dim3(1), dim3(64), 0, cst, total, d_out32, d_outjac);
+    if (g_horner_quad)
+        hipLaunchKernelGGL(hs::k_finalize<C>, dim3(1), dim3(64), 0, cst, total, d_out32, d_outjac);
     if (flags & HS_GPU_ERASE)
     {
         // the digits and the sorted entries (point index and digit sign) are functions of the scalars
@@ -743,12 +748,14 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
     hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((g.V + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, g.B / g.G, 0u,
                        g.V, win);
     if (g_horner_quad)
+    {
         hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((4 * g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
                            g.total.as<hs::jac>());
+        hipLaunchKernelGGL(hs::k_finalize_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, g.total.as<hs::jac>(), d,
+                           g.M);
+    }
     else
-        hipLaunchKernelGGL(hs::k_horner_batch_row<C>, dim3(g.M), dim3(64), 0, st, win, g.M, g.W, g.c,
-                           g.total.as<hs::jac>());
-    hipLaunchKernelGGL(hs::k_finalize_batch<C>, dim3((g.M + 63) / 64), dim3(64), 0, st, g.total.as<hs::jac>(), d, g.M);
+        hipLaunchKernelGGL(hs::k_horner_batch_row<C>, dim3(g.M), dim3(64), 0, st, win, g.M, g.W, g.c, d);
     HS_CHECK(hipGetLastError());
     return 0;
 }

@@ -741,33 +741,12 @@ __global__ void __launch_bounds__(64) k_horner(const jac *__restrict__ win_out, 
         *total = acc;
 }
 
-// The same Horner step with every field element spread over a DPP row (rowcurve.h): one whole
-// wavefront, four products per formula level.
-template <class C>
-__global__ void __launch_bounds__(64) k_horner_row(const jac *__restrict__ win_out, int w_lo, int w_hi, int c,
-                                                   int first, jac *total)
-{
-    using F = typename C::F;
-    row::RowK<F> k;
-    row::rowk_init(k);
-    row::rjac acc;
-    row::rhorner<F>(k, acc, win_out, w_lo, w_hi, c, first != 0, total);
-    jac r;
-    row::rjac_get<F>(r, acc);
-    if (threadIdx.x == 0)
-        *total = r;
-}
-
 // out32: compressed point (8 words); out_jac: reference Jacobian struct (15 uint64): (x : y : 1)
-// canonical, or (1 : 1 : 0) for the identity (helios_ops.h:44-50).
+// canonical, or (1 : 1 : 0) for the identity (helios_ops.h:44-50).  Runs on one lane.
 template <class C>
-__global__ void __launch_bounds__(64) k_finalize(const jac *__restrict__ total, uint32_t *__restrict__ out32,
-                                                 uint64_t *__restrict__ out_jac)
+__device__ __forceinline__ void finalize_point(const jac &acc, uint32_t *__restrict__ out32, uint64_t *__restrict__ out_jac)
 {
     using F = typename C::F;
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
-    jac acc = *total;
     aff a;
     bool pt = jac_to_aff<F>(a, acc);
     uint32_t enc[8];
@@ -795,6 +774,42 @@ __global__ void __launch_bounds__(64) k_finalize(const jac *__restrict__ total, 
                 out_jac[k] = (k == 0 || k == 5) ? 1u : 0u;
         }
     }
+}
+
+template <class C>
+__global__ void __launch_bounds__(64) k_finalize(const jac *__restrict__ total, uint32_t *__restrict__ out32,
+                                                 uint64_t *__restrict__ out_jac)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    jac acc = *total;
+    finalize_point<C>(acc, out32, out_jac);
+}
+
+// The Horner step with every field element spread over a DPP row (rowcurve.h): one whole
+// wavefront, four products per formula level.  With an output (out32 or out_jac) the kernel also
+// finalizes (affine conversion and encoding, on lane 0) instead of storing the total.
+template <class C>
+__global__ void __launch_bounds__(64) k_horner_row(const jac *__restrict__ win_out, int w_lo, int w_hi, int c,
+                                                   int first, jac *total, uint32_t *__restrict__ out32,
+                                                   uint64_t *__restrict__ out_jac)
+{
+    using F = typename C::F;
+    row::RowK<F> k;
+    row::rowk_init(k);
+    row::rjac acc;
+    row::rhorner<F>(k, acc, win_out, w_lo, w_hi, c, first != 0, total);
+    jac r;
+    row::rjac_get<F>(r, acc);
+    if (threadIdx.x != 0)
+        return;
+    if (!out32 && !out_jac)
+    {
+        *total = r;
+        return;
+    }
+    force_vgpr_jac(r); // the finalization of a wave-uniform value would otherwise run on the scalar unit
+    finalize_point<C>(r, out32, out_jac);
 }
 
 } // namespace hs
@@ -1149,10 +1164,11 @@ __global__ void __launch_bounds__(64) k_horner_batch(const jac *__restrict__ win
         total[m] = acc;
 }
 
-// Horner per MSM with the row-distributed arithmetic: one wavefront (workgroup) per MSM.
+// Horner per MSM with the row-distributed arithmetic, one wavefront (workgroup) per MSM, then the
+// MSM's affine conversion and encoding on lane 0 (k_finalize_batch's work).
 template <class C>
 __global__ void __launch_bounds__(64) k_horner_batch_row(const jac *__restrict__ win_out, uint32_t M, int W, int c,
-                                                         jac *__restrict__ total)
+                                                         const BatchDesc *__restrict__ d)
 {
     using F = typename C::F;
     const uint32_t m = blockIdx.x;
@@ -1164,8 +1180,18 @@ __global__ void __launch_bounds__(64) k_horner_batch_row(const jac *__restrict__
     row::rhorner<F>(k, acc, win_out + (size_t)m * W, 0, W, c, true, nullptr);
     jac r;
     row::rjac_get<F>(r, acc);
-    if (threadIdx.x == 0)
-        total[m] = r;
+    if (threadIdx.x != 0)
+        return;
+    force_vgpr_jac(r);
+    aff a;
+    const bool pt = jac_to_aff<F>(a, r);
+    uint32_t enc[8];
+    encode_aff<F>(enc, a, pt);
+    if (d[m].out32)
+        for (int j = 0; j < 8; j++)
+            d[m].out32[j] = enc[j];
+    if (d[m].outjac)
+        store_r51_aff<F>(d[m].outjac, a, pt);
 }
 
 template <class C>
