@@ -627,9 +627,10 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         const uint32_t nseg = (uint32_t)(w_hi - w_lo) * segs_per_win;
         hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((nseg + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G,
                            (uint32_t)w_lo, nseg, seg);
-        const uint32_t nwin = (uint32_t)(w_hi - w_lo), wpb = reduce_win_per_block(segs_per_win);
-        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((nwin + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg,
-                           segs_per_win, (uint32_t)w_lo, nwin, win);
+        const uint32_t parts = hs::seg_partials(segs_per_win);
+        const uint32_t nwin = (uint32_t)(w_hi - w_lo), wpb = reduce_win_per_block(parts);
+        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((nwin + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, parts,
+                           (uint32_t)w_lo, nwin, win);
         cx.mark(st, 4);
         if (Q > 1)
         {
@@ -744,8 +745,8 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
                        g.bnd.as<hs::xyzz>(), g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>(), buckets);
     hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((g.nseg + 255) / 256), blk, 0, st, buckets, offsets, g.B, g.G, 0u, g.nseg,
                        seg);
-    const uint32_t wpb = reduce_win_per_block(g.B / g.G);
-    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((g.V + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, g.B / g.G, 0u,
+    const uint32_t parts = hs::seg_partials(g.B / g.G), wpb = reduce_win_per_block(parts);
+    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((g.V + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, parts, 0u,
                        g.V, win);
     if (g_horner_quad)
     {

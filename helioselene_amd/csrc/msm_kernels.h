@@ -601,36 +601,65 @@ __global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__re
 // Window w, segment s covers buckets [s*G, s*G + G) (bucket b holds digit magnitude b + 1).
 // Running sums give T = sum (b - s*G + 1) B_b and R = sum B_b; the segment's share of
 // sum (b+1) B_b is T + (s*G) R (reference running-sum reduction, helios_msm_vartime.cpp:334-361).
+// The workgroup then adds its threads' shares over groups of tw = min(segs_per_win, 256)
+// consecutive segments (one window each) with an LDS tree: one partial sum per group, at index
+// (global segment index) / tw, so a window leaves segs_per_win / tw partials for k_reduce_win.
 template <class C>
 __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buckets, const uint32_t *__restrict__ offsets,
                                                     uint32_t B, uint32_t G, uint32_t w_lo, uint32_t nseg,
                                                     xyzz *__restrict__ seg_out)
 {
     using F = typename C::F;
+    __shared__ xyzz sh[256];
     const uint32_t segs_per_win = B / G;
-    uint32_t t = w_lo * segs_per_win + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= w_lo * segs_per_win + nseg)
-        return;
-    const uint32_t w = t / segs_per_win, s = t % segs_per_win;
-    const size_t g0 = (size_t)w * B + (size_t)s * G;
-    xyzz R, T;
-    xyzz_identity<F>(R);
+    const uint32_t t = w_lo * segs_per_win + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < w_lo * segs_per_win + nseg;
+    xyzz T;
     xyzz_identity<F>(T);
-    for (int j = (int)G - 1; j >= 0; j--)
+    if (live)
     {
-        if (offsets[g0 + j] != offsets[g0 + j + 1]) // empty buckets hold the identity
+        const uint32_t w = t / segs_per_win, s = t % segs_per_win;
+        const size_t g0 = (size_t)w * B + (size_t)s * G;
+        xyzz R;
+        xyzz_identity<F>(R);
+        for (int j = (int)G - 1; j >= 0; j--)
         {
-            xyzz b;
-            load_xyzz(b, buckets + g0 + j);
-            xyzz_add<F>(R, R, b);
+            if (offsets[g0 + j] != offsets[g0 + j + 1]) // empty buckets hold the identity
+            {
+                xyzz b;
+                load_xyzz(b, buckets + g0 + j);
+                xyzz_add<F>(R, R, b);
+            }
+            xyzz_add<F>(T, T, R);
         }
-        xyzz_add<F>(T, T, R);
+        xyzz U;
+        xyzz_mul_small<F>(U, R, s * G);
+        xyzz_add<F>(T, T, U);
     }
-    xyzz U;
-    xyzz_mul_small<F>(U, R, s * G);
-    xyzz_add<F>(T, T, U);
-    store_xyzz(seg_out + t, T);
+    const uint32_t tw = segs_per_win < 256u ? segs_per_win : 256u, lt = threadIdx.x % tw;
+    store_xyzz(sh + threadIdx.x, T);
+    __syncthreads();
+    for (uint32_t h = tw >> 1; h > 0; h >>= 1)
+    {
+        if (lt < h)
+        {
+            xyzz x, y;
+            load_xyzz(x, sh + threadIdx.x);
+            load_xyzz(y, sh + threadIdx.x + h);
+            xyzz_add<F>(x, x, y);
+            store_xyzz(sh + threadIdx.x, x);
+        }
+        __syncthreads();
+    }
+    if (lt == 0 && live)
+    {
+        load_xyzz(T, sh + threadIdx.x);
+        store_xyzz(seg_out + t / tw, T);
+    }
 }
+
+// partials per window that k_reduce_seg leaves
+__host__ __device__ inline uint32_t seg_partials(uint32_t segs_per_win) { return segs_per_win < 256u ? 1u : segs_per_win / 256u; }
 
 // Sums of the windows' segment results.  A window of segs_per_win segments (a power of two) gets
 // tw = min(segs_per_win, 256) threads, so one workgroup covers 256 / tw windows: each thread adds

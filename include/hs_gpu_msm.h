@@ -297,7 +297,7 @@ int hs_gpu_probe_int_mul_peak(double *products_per_second);
  * rate of the multiply is compute_units x 4 SIMDs x 16 lanes x clock. */
 int hs_gpu_device_info(int *compute_units, int *max_clock_khz, char *name, size_t name_cap);
 /* Copies intermediate state of the last plain MSM on this device (inspection in tests): which = 0
- * window sums, 1 segment sums, 2 bucket sums, 5 partial sums of the bucket runs cut by chunk
+ * window sums, 1 segment sums (per window, partials over groups of up to 256 segments), 2 bucket sums, 5 partial sums of the bucket runs cut by chunk
  * edges (as 32-byte encodings), 3 sorted bucket entries, 4 bucket offsets (uint32).  *count receives the number of items written. */
 int hs_gpu_debug_dump(int curve, int which, void *out, size_t cap_bytes, size_t *count);
 
