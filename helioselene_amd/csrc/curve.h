@@ -151,6 +151,87 @@ template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
     acc.Y = Y3;
 }
 
+// acc = a or -a as (x, y, 1, +-1).  For -a this is the representation with lambda = -1
+// (ZZ = lambda^2 = 1, ZZZ = lambda^3 = -1, so y = Y / ZZZ = -y_a): a run of the bucket walk starts
+// from its first entry's base without negating y.
+template <class F> HS_HD void xyzz_from_aff_signed(xyzz &r, const aff &a, bool neg)
+{
+    r.X = a.x;
+    r.Y = a.y;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        r.ZZ.v[i] = i == 0 ? 1u : 0u;
+        r.ZZZ.v[i] = neg ? F::mod(i) - (i == 0 ? 1u : 0u) : (i == 0 ? 1u : 0u);
+    }
+}
+
+// The same addition for the bucket walk, laid out for SIMD lanes that nearly always take the
+// common path: the madd-2008-s result is formed unconditionally and written to acc; the rare cases
+// (acc was the identity, acc == q, acc == -q) are fixed up afterwards in a branch the whole
+// wavefront takes only when one of its lanes needs it, and the fix-up selects limb by limb.  A
+// divergent branch that redefines acc would make the compiler keep the common result in other
+// registers and copy it back on every step.
+template <class F> HS_HD void xyzz_madd_hot(xyzz &acc, bool &acc_id, const aff &q)
+{
+    fe nX, nY, P, R;
+    fe_neg_lazy2<F>(nX, acc.X);
+    fe_neg_lazy2<F>(nY, acc.Y);
+    fe_mul_add<F>(P, q.x, acc.ZZ, nX);
+    fe_mul_add<F>(R, q.y, acc.ZZZ, nY);
+    fe_opaque(P);
+    fe_opaque(R);
+    // only the cheap exclusion runs here; the canonical zero test waits in the rare branch
+    const bool rare = acc_id || fe_maybe_zero_norm(P);
+    fe PP, PPP, Q, t;
+    fe_sq<F>(PP, P);
+    fe_mul<F>(PPP, P, PP);
+    fe_mul<F>(Q, acc.X, PP);
+    fe X3, Y3;
+    fe_neg3_lazy<F>(t, PPP, Q, Q);
+    fe_sq_add<F>(X3, R, t); // R^2 - PPP - 2Q
+    fe_sub_lazy2<F>(t, Q, X3);
+    fe_mul_sum2<F>(Y3, R, t, nY, PPP); // R (Q - X3) - Y1 PPP
+    fe_mul<F>(acc.ZZ, acc.ZZ, PP);
+    fe_mul<F>(acc.ZZZ, acc.ZZZ, PPP);
+    acc.X = X3;
+    acc.Y = Y3;
+#ifdef __HIP_DEVICE_COMPILE__
+    const bool any_rare = __builtin_amdgcn_ballot_w64(rare) != 0;
+#else
+    const bool any_rare = rare;
+#endif
+    if (any_rare)
+    {
+        // the empty asm pins these copies inside the branch, so nothing computed from them is
+        // hoisted onto the common path
+        aff d = q;
+        fe_opaque(d.x);
+        fe_opaque(d.y);
+        fe_opaque(P);
+        fe_opaque(R);
+        fe_norm<F>(d.y);
+        const bool zero_p = rare && !acc_id && fe_is_zero_canon<F>(P);
+        const bool same = zero_p && fe_is_zero_norm<F>(R);
+        xyzz alt;
+        if (same)
+            xyzz_mdbl<F>(alt, d); // acc == q
+        else
+            xyzz_from_aff<F>(alt, d);
+        const bool use = same || (rare && acc_id);
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+        {
+            acc.X.v[i] = use ? alt.X.v[i] : acc.X.v[i];
+            acc.Y.v[i] = use ? alt.Y.v[i] : acc.Y.v[i];
+            acc.ZZ.v[i] = use ? alt.ZZ.v[i] : acc.ZZ.v[i];
+            acc.ZZZ.v[i] = use ? alt.ZZZ.v[i] : acc.ZZZ.v[i];
+        }
+        // acc == -q leaves the identity; a start from the identity leaves q
+        acc_id = (zero_p && !same) ? true : (acc_id ? false : acc_id);
+    }
+}
+
 // r = p + q (both xyzz), add-2008-s, with identity / doubling / inverse handling.
 template <class F> HS_HD void xyzz_add(xyzz &r, const xyzz &p, const xyzz &q)
 {

@@ -683,6 +683,14 @@ template <class F> HS_HD bool fe_is_zero_norm(const fe &a)
     return fe_is_zero_canon<F>(a);
 }
 
+// The cheap half of fe_is_zero_norm: false proves the value nonzero; true leaves the canonical test
+// (fe_is_zero_canon) to decide.
+HS_HD bool fe_maybe_zero_norm(const fe &a)
+{
+    const uint32_t o = a.v[5] | a.v[6] | a.v[7], n = a.v[5] & a.v[6] & a.v[7];
+    return o == 0 || n == M29;
+}
+
 template <class F> HS_HD bool fe_eq(const fe &a, const fe &b)
 {
     fe d;

@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(SORT_FINE_THREADS) k_fine_sort(const uint32_t 
 // ---------------------------------------------------------------- bucket sums
 constexpr uint32_t ENTRY_SKIP = 0xffffffffu; // padding entry of the batched layout
 
-// Raw 64-byte base of an entry (point 0 for padding, which add_entry_raw then skips); issued one
+// Raw 64-byte base of an entry (point 0 for padding, which the walk then skips); issued one
 // entry ahead in the flat kernel so the gather overlaps the current addition.
 __device__ __forceinline__ void load_entry_raw(uint4 (&q)[4], const uint32_t *__restrict__ pts, uint32_t e)
 {
@@ -398,12 +398,11 @@ __device__ __forceinline__ void load_entry_raw(uint4 (&q)[4], const uint32_t *__
     q[3] = p[3];
 }
 
+// acc += the entry's base (or its negation).  The caller passes no padding entry.
 template <class C>
 __device__ __forceinline__ void add_entry_raw(xyzz &acc, bool &acc_id, const uint4 (&q)[4], uint32_t e)
 {
     using F = typename C::F;
-    if (e == ENTRY_SKIP)
-        return;
     const uint32_t xw[8] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w};
     const uint32_t yw[8] = {q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
     aff p;
@@ -414,7 +413,7 @@ __device__ __forceinline__ void add_entry_raw(xyzz &acc, bool &acc_id, const uin
 #pragma unroll
     for (int i = 0; i < NL; i++)
         p.y.v[i] = (p.y.v[i] & keep) | ((F::bias2(i) - p.y.v[i]) & ~keep);
-    xyzz_madd<F>(acc, acc_id, p);
+    xyzz_madd_hot<F>(acc, acc_id, p);
 }
 
 constexpr int BIG_THREADS = 256;
@@ -467,6 +466,84 @@ __global__ void __launch_bounds__(256) k_chunk_start(const uint32_t *__restrict_
         chunk_bucket[t] = g;
 }
 
+// A run of one bucket starts from its first entry's base, +-P as (x, y, 1, +-1)
+// (xyzz_from_aff_signed); a padding entry starts it as the identity.
+template <class F>
+__device__ __forceinline__ void run_start_raw(xyzz &acc, bool &acc_id, const uint4 (&q)[4], uint32_t e)
+{
+    const uint32_t xw[8] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w};
+    const uint32_t yw[8] = {q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
+    aff p;
+    fe_from_words(p.x, xw);
+    fe_from_words(p.y, yw);
+    xyzz_from_aff_signed<F>(acc, p, (e >> 31) != 0);
+    acc_id = e == ENTRY_SKIP;
+}
+
+// State of one thread's walk over its chunk [pos, end) of sorted entries (k_bucket_flat).
+struct FlatWalk
+{
+    uint32_t pos, end, g, g_hi, bucket_end, next_end, t, e_next;
+    bool first, first_whole;
+};
+
+// One step of the walk: entry `pos` (index e, record in `cur`) either extends the current run by
+// a mixed addition or, when the run has ended at pos, the run is stored and a new one starts with
+// it.  The record of entry pos + 1 is issued into `nxt` first, so its gather overlaps the
+// addition.  Returns false once the chunk is done (the last run stored).
+template <class C>
+__device__ __forceinline__ bool flat_step(FlatWalk &f, xyzz &acc, bool &acc_id, uint32_t &e, const uint4 (&cur)[4],
+                                          uint4 (&nxt)[4], const uint32_t *__restrict__ offsets,
+                                          const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ pts,
+                                          xyzz *__restrict__ buckets, xyzz *__restrict__ bnd)
+{
+    using F = typename C::F;
+    const uint32_t e_cur = e;
+    if (f.pos + 1 < f.end)
+    {
+        e = f.e_next;
+        load_entry_raw(nxt, pts, e);
+        if (f.pos + 2 < f.end)
+            f.e_next = sorted[f.pos + 2];
+    }
+    const bool run_done = f.pos == f.bucket_end || f.pos == f.end;
+    // padding entries (batched layouts put them at the end of a window's last bucket) add nothing
+    if (run_done || e_cur == ENTRY_SKIP)
+    {
+        if (!run_done)
+        {
+            f.pos++;
+            return true;
+        }
+        if (acc_id)
+            xyzz_identity<F>(acc);
+        // later runs start at their bucket's first entry, so only the first run can be partial there
+        const bool whole = f.pos == f.bucket_end && (!f.first || f.first_whole);
+        if (whole)
+            store_xyzz(buckets + f.g, acc);
+        else
+            store_xyzz(bnd + 2 * (size_t)f.t + (f.first ? 0 : 1), acc);
+        if (f.pos == f.end)
+            return false;
+        f.first = false;
+        // next non-empty bucket (entries are contiguous, so it starts at pos); its end was
+        // prefetched, so only empty buckets cost a dependent load here
+        f.g++;
+        f.bucket_end = f.next_end;
+        while (f.bucket_end == f.pos)
+        {
+            f.g++;
+            f.bucket_end = offsets[f.g + 1];
+        }
+        f.next_end = f.g + 2 <= f.g_hi ? offsets[f.g + 2] : f.end;
+        run_start_raw<F>(acc, acc_id, cur, e_cur);
+    }
+    else
+        add_entry_raw<C>(acc, acc_id, cur, e_cur);
+    f.pos++;
+    return true;
+}
+
 template <class C>
 __global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict__ offsets,
                                                      const uint32_t *__restrict__ sorted, uint32_t g_lo, uint32_t g_hi,
@@ -480,55 +557,39 @@ __global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict_
     const uint32_t E = offsets[g_hi], beg = offsets[g_lo] + t * L;
     if (beg >= E)
         return;
-    const uint32_t end = min(E, beg + L);
-    uint32_t g = chunk_bucket[t];
-    uint32_t run_start = beg, bucket_end = offsets[g + 1];
-    bool first = true;
+    FlatWalk f;
+    f.t = t;
+    f.end = min(E, beg + L);
+    f.g_hi = g_hi;
+    f.g = chunk_bucket[t];
+    f.bucket_end = offsets[f.g + 1];
+    f.next_end = f.g + 2 <= g_hi ? offsets[f.g + 2] : f.end;
+    f.first = true;
+    f.first_whole = offsets[f.g] == beg;
+    // the first entry starts the first run; the walk then begins at beg + 1
+    uint4 qa[4], qb[4];
+    uint32_t e = sorted[beg];
+    load_entry_raw(qa, pts, e);
+    f.e_next = beg + 1 < f.end ? sorted[beg + 1] : ENTRY_SKIP;
     xyzz acc;
-    bool acc_id = true;
-    xyzz_identity<F>(acc);
-    uint32_t pos = beg;
-    uint32_t e_cur = sorted[pos];
-    uint4 q[4];
-    load_entry_raw(q, pts, e_cur);
-    uint32_t e_next = pos + 1 < end ? sorted[pos + 1] : ENTRY_SKIP;
+    bool acc_id;
+    run_start_raw<F>(acc, acc_id, qa, e);
+    f.pos = beg + 1;
+    // entry beg + 1's record goes to qb; from then on the two buffers alternate (unrolled by two,
+    // so the in-flight record never needs a register copy)
+    if (f.pos < f.end)
+    {
+        e = f.e_next;
+        load_entry_raw(qb, pts, e);
+        if (f.pos + 1 < f.end)
+            f.e_next = sorted[f.pos + 1];
+    }
     while (true)
     {
-        const uint32_t e = e_cur;
-        const uint4 cq[4] = {q[0], q[1], q[2], q[3]};
-        pos++;
-        if (pos < end)
-        {
-            // the next entry's base and the entry after it are in flight during this addition
-            e_cur = e_next;
-            load_entry_raw(q, pts, e_cur);
-            if (pos + 1 < end)
-                e_next = sorted[pos + 1];
-        }
-        add_entry_raw<C>(acc, acc_id, cq, e);
-        const bool run_done = pos == bucket_end || pos == end;
-        if (run_done)
-        {
-            if (acc_id)
-                xyzz_identity<F>(acc);
-            const bool whole = run_start == offsets[g] && pos == bucket_end;
-            if (whole)
-                store_xyzz(buckets + g, acc);
-            else
-                store_xyzz(bnd + 2 * (size_t)t + (first ? 0 : 1), acc);
-            if (pos == end)
-                break;
-            first = false;
-            // next non-empty bucket (entries are contiguous, so it starts at pos)
-            do
-            {
-                g++;
-                bucket_end = offsets[g + 1];
-            } while (bucket_end == pos);
-            run_start = pos;
-            acc_id = true;
-            xyzz_identity<F>(acc);
-        }
+        if (!flat_step<C>(f, acc, acc_id, e, qb, qa, offsets, sorted, pts, buckets, bnd))
+            break;
+        if (!flat_step<C>(f, acc, acc_id, e, qa, qb, offsets, sorted, pts, buckets, bnd))
+            break;
     }
 }
 

@@ -315,6 +315,53 @@ template <class F> static void madd_chain(uint8_t *out, const uint8_t *pts, cons
     out[64] = ok;
 }
 
+// The bucket walk's form of the same sum: the first non-identity point starts the run as
+// (x, y, 1, +-1) (xyzz_from_aff_signed, y not negated), the rest go through xyzz_madd_hot with a
+// lazily negated y, as k_bucket_flat does.
+template <class F> static void walk_chain(uint8_t *out, const uint8_t *pts, const uint8_t *neg, size_t n)
+{
+    xyzz acc;
+    bool aid = true, started = false;
+    xyzz_identity<F>(acc);
+    for (size_t i = 0; i < n; i++)
+    {
+        int nz = 0;
+        for (int j = 0; j < 64; j++)
+            nz |= pts[64 * i + j];
+        if (!nz)
+            continue;
+        aff q;
+        ld(q.x, pts + 64 * i);
+        ld(q.y, pts + 64 * i + 32);
+        if (!started)
+        {
+            xyzz_from_aff_signed<F>(acc, q, neg[i] != 0);
+            aid = false;
+            started = true;
+            continue;
+        }
+        if (neg[i])
+            fe_neg_lazy2<F>(q.y, q.y);
+        xyzz_madd_hot<F>(acc, aid, q);
+    }
+    if (aid)
+        xyzz_identity<F>(acc);
+    aff r;
+    bool ok = xyzz_to_aff<F>(r, acc);
+    st(out, r.x);
+    st(out + 32, r.y);
+    out[64] = ok;
+}
+
+extern "C" void hst_walk_chain(int field, uint8_t *out, const uint8_t *pts, const uint8_t *neg, size_t n)
+{
+    g_field = field;
+    if (field == 0)
+        walk_chain<Fp>(out, pts, neg, n);
+    else
+        walk_chain<Fq>(out, pts, neg, n);
+}
+
 extern "C" void hst_madd_chain(int field, uint8_t *out, const uint8_t *pts, const uint8_t *neg, size_t n)
 {
     g_field = field;

@@ -119,10 +119,13 @@ def test_point_ops_vs_oracle(L, c):
         assert _enc(out.raw) == O.scalar_mul(c, k.to_bytes(32, "little"), P)
 
 
+@pytest.mark.parametrize("fn", ["hst_madd_chain", "hst_walk_chain"])
 @pytest.mark.parametrize("c", [0, 1])
-def test_madd_chain_vs_oracle(L, c):
-    """Long xyzz_madd chains (the bucket accumulation's inner loop) with repeated points, negated
-    points, identities and cancellations, against the oracle's sum."""
+def test_madd_chain_vs_oracle(L, c, fn):
+    """Long xyzz_madd chains with repeated points, negated points, identities and cancellations,
+    against the oracle's sum.  hst_walk_chain is the bucket walk's form: the run starts from its
+    first point as (x, y, 1, +-1) and continues with xyzz_madd_hot; a negated first point (trial 2)
+    exercises the ZZZ = -1 start."""
     rnd = random.Random(40 + c)
     g = O.generator(c)
     base = [O.scalar_mul(c, rnd.getrandbits(250).to_bytes(32, "little"), g) for _ in range(40)]
@@ -135,6 +138,11 @@ def test_madd_chain_vs_oracle(L, c):
             idx[10:14] = [idx[9]] * 4
             neg[10:14] = [neg[9]] * 4
             idx[50], neg[50] = idx[49], not neg[49]
+        if trial == 2:
+            neg[0] = True
+        if trial == 3:
+            # the first point cancelled by the second: the run continues from the identity
+            idx[1], neg[0], neg[1] = idx[0], True, False
         pts, want_sc = bytearray(), bytearray()
         for i in range(n):
             P = base[idx[i]]
@@ -148,8 +156,9 @@ def test_madd_chain_vs_oracle(L, c):
         order = O.ORDER[c]
         want = O.msm_naive(c, b"".join((v % order).to_bytes(32, "little") for v in sc), b"".join(base), len(base))
         out = ctypes.create_string_buffer(65)
-        L.hst_madd_chain.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
-        L.hst_madd_chain(c, out, bytes(pts), bytes(int(v) for v in neg), n)
+        f = getattr(L, fn)
+        f.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+        f(c, out, bytes(pts), bytes(int(v) for v in neg), n)
         assert _enc(out.raw) == want, trial
 
 
