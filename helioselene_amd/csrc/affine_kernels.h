@@ -157,9 +157,9 @@ __global__ void __launch_bounds__(64) k_bucket_affine(const uint32_t *__restrict
             const uint32_t e = sorted[c.pos];
             if (e == ENTRY_SKIP)
                 continue;
-            load_aff(p[j], pts, e & 0x7fffffffu);
-            if (e >> 31)
-                fe_neg<F>(p[j].y, p[j].y);
+            EntryBase b;
+            load_entry_base(b, pts, e);
+            entry_point(p[j], b);
             if (c.acc_id)
             {
                 kinds |= AFF_START << (3 * j);

@@ -151,21 +151,6 @@ template <class F> HS_HD void xyzz_madd(xyzz &acc, bool &acc_id, const aff &q)
     acc.Y = Y3;
 }
 
-// acc = a or -a as (x, y, 1, +-1).  For -a this is the representation with lambda = -1
-// (ZZ = lambda^2 = 1, ZZZ = lambda^3 = -1, so y = Y / ZZZ = -y_a): a run of the bucket walk starts
-// from its first entry's base without negating y.
-template <class F> HS_HD void xyzz_from_aff_signed(xyzz &r, const aff &a, bool neg)
-{
-    r.X = a.x;
-    r.Y = a.y;
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-    {
-        r.ZZ.v[i] = i == 0 ? 1u : 0u;
-        r.ZZZ.v[i] = neg ? F::mod(i) - (i == 0 ? 1u : 0u) : (i == 0 ? 1u : 0u);
-    }
-}
-
 // The same addition for the bucket walk, laid out for SIMD lanes that nearly always take the
 // common path: the madd-2008-s result is formed unconditionally and written to acc; the rare cases
 // (acc was the identity, acc == q, acc == -q) are fixed up afterwards in a branch the whole

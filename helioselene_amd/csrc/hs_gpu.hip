@@ -139,7 +139,7 @@ void zero_dev(void *p, size_t bytes, hipStream_t st)
 struct Plan
 {
     int c, W, Q;
-    uint32_t B, G, nb, segs, L;
+    uint32_t B, G, nb, L;
     int affK = 0; // bucket method: 0 = xyzz mixed additions (k_bucket_flat), K = batched affine with K chains per lane
 };
 
@@ -248,6 +248,18 @@ bool tune_lookup(int mode, int curve, size_t n, TuneEntry &out)
     return true;
 }
 
+// Buckets per reduction segment: a power of two in [B / MAX_SEGS_PER_WIN, B] (the window
+// reduction doubles its subset sums log2 G times and keeps at most MAX_SEGS_PER_WIN / 256 blocks of
+// a window in LDS).
+static uint32_t plan_G(uint32_t B, uint32_t G)
+{
+    uint32_t p = 1;
+    while (2 * p <= G)
+        p *= 2;
+    p = std::min(p, B);
+    return std::max(p, B / hs::MAX_SEGS_PER_WIN > 0 ? B / hs::MAX_SEGS_PER_WIN : 1u);
+}
+
 Plan make_plan(int curve, size_t n)
 {
     Plan p;
@@ -260,8 +272,7 @@ Plan make_plan(int curve, size_t n)
         p.c = 16;
     p.W = (256 + p.c - 1) / p.c;
     p.B = 1u << (p.c - 1);
-    uint32_t g = (uint32_t)env_int("HS_GPU_G", tuned ? te.G : 8);
-    p.G = p.B < g ? p.B : g;
+    p.G = plan_G(p.B, (uint32_t)env_int("HS_GPU_G", tuned ? te.G : 8));
     p.nb = (uint32_t)p.W * p.B;
     p.L = (uint32_t)env_int("HS_GPU_L", tuned ? te.L : 32);
     if (p.L < 1)
@@ -273,7 +284,6 @@ Plan make_plan(int curve, size_t n)
     p.affK = env_int("HS_GPU_AFFINE_K", 0);
     if (p.affK != 1 && p.affK != 2 && p.affK != 4)
         p.affK = 0;
-    p.segs = p.nb / p.G;
     return p;
 }
 
@@ -470,10 +480,6 @@ int scan_exclusive(Workspace &cx, const uint32_t *in, uint32_t *out, uint32_t le
 }
 
 // windows per k_reduce_win workgroup (256 threads, min(segments, 256) of them per window)
-static uint32_t reduce_win_per_block(uint32_t segs_per_win)
-{
-    return segs_per_win < (uint32_t)hs::BIG_THREADS ? (uint32_t)hs::BIG_THREADS / segs_per_win : 1u;
-}
 
 // The plain MSM on device-resident inputs.  d_scalars: n x 8 words; d_points: n x 15 uint64.
 // Point indices in the sort entries have 24 bits: larger MSMs run as chunks (msm_device_any).
@@ -510,7 +516,7 @@ MsmGeom msm_geom(const Plan &P, size_t n)
 std::vector<std::pair<DevBuf *, size_t>> msm_buffers(Workspace &ws, const Plan &P, const MsmGeom &g, size_t n)
 {
     const uint32_t per = hs::SCAN_BLOCK * hs::SCAN_ITEMS, sblk = (g.scan_len + per - 1) / per;
-    return {{&ws.aff, n * 64},
+    return {{&ws.aff, n * hs::BASE_RECORD_BYTES},
             {&ws.isid, n},
             {&ws.digits, n * 2 * (size_t)P.W},
             {&ws.offsets, (size_t)(P.nb + 1) * 4},
@@ -519,7 +525,7 @@ std::vector<std::pair<DevBuf *, size_t>> msm_buffers(Workspace &ws, const Plan &
             {&ws.off_a, (size_t)g.scan_len * 4},
             {&ws.sorted, n * 4 * (size_t)P.W + 4},
             {&ws.buckets, (size_t)P.nb * sizeof(hs::xyzz)},
-            {&ws.seg, (size_t)P.segs * sizeof(hs::xyzz)},
+            {&ws.seg, hs::seg_out_entries(P.W, P.B / P.G) * sizeof(hs::xyzz)},
             {&ws.win, (size_t)P.W * sizeof(hs::xyzz)},
             {&ws.big_list, (size_t)P.nb * 4},
             {&ws.big_count, 64},
@@ -627,10 +633,8 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         const uint32_t nseg = (uint32_t)(w_hi - w_lo) * segs_per_win;
         hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((nseg + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G,
                            (uint32_t)w_lo, nseg, seg);
-        const uint32_t parts = hs::seg_partials(segs_per_win);
-        const uint32_t nwin = (uint32_t)(w_hi - w_lo), wpb = reduce_win_per_block(parts);
-        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((nwin + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, parts,
-                           (uint32_t)w_lo, nwin, win);
+        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((uint32_t)(w_hi - w_lo)), dim3(256), hs::reduce_win_lds(segs_per_win),
+                           st, seg, segs_per_win, P.G, (uint32_t)w_lo, win);
         cx.mark(st, 4);
         if (Q > 1)
         {
@@ -745,9 +749,8 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
                        g.bnd.as<hs::xyzz>(), g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>(), buckets);
     hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((g.nseg + 255) / 256), blk, 0, st, buckets, offsets, g.B, g.G, 0u, g.nseg,
                        seg);
-    const uint32_t parts = hs::seg_partials(g.B / g.G), wpb = reduce_win_per_block(parts);
-    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((g.V + wpb - 1) / wpb), dim3(hs::BIG_THREADS), 0, st, seg, parts, 0u,
-                       g.V, win);
+    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(g.V), dim3(256), hs::reduce_win_lds(g.B / g.G), st, seg, g.B / g.G, g.G,
+                       0u, win);
     if (g_horner_quad)
     {
         hipLaunchKernelGGL(hs::k_horner_batch<C>, dim3((4 * g.M + 63) / 64), dim3(64), 0, st, win, g.M, g.W, g.c,
@@ -917,7 +920,7 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
     }
     for (BatchGroup *g : b->groups)
     {
-        g->G = std::max(1u, std::min(g->B, g->G));
+        g->G = plan_G(g->B, g->G);
         g->L = std::max(1u, g->L);
         g->M = (uint32_t)g->hdesc.size();
         g->V = g->M * (uint32_t)g->W;
@@ -927,13 +930,13 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
         const size_t E = (size_t)g->N * g->W;
         const std::pair<DevBuf *, size_t> bufs[] = {
             {&g->desc, g->M * sizeof(hs::BatchDesc)},
-            {&g->aff, (size_t)g->N * 64},
+            {&g->aff, (size_t)g->N * hs::BASE_RECORD_BYTES},
             {&g->isid, g->N},
             {&g->digits, E * 2},
             {&g->sorted, E * 4 + 4},
             {&g->offsets, ((size_t)g->nb + 1) * 4},
             {&g->buckets, (size_t)g->nb * sizeof(hs::xyzz)},
-            {&g->seg, (size_t)g->nseg * sizeof(hs::xyzz)},
+            {&g->seg, hs::seg_out_entries(g->V, g->B / g->G) * sizeof(hs::xyzz)},
             {&g->win, (size_t)g->V * sizeof(hs::xyzz)},
             {&g->total, (size_t)g->M * sizeof(hs::jac)},
             {&g->chunk_bucket, g->chunks * 4 + 4},
@@ -1628,10 +1631,10 @@ size_t request_footprint(int curve, size_t n, bool device_inputs)
     int c = tuned ? te.c : choose_c_batch(n);
     c = c < 4 ? 4 : c > 13 ? 13 : c;
     const size_t W = (256 + c - 1) / c, B = (size_t)1 << (c - 1);
-    const size_t G = std::min(B, (size_t)(tuned ? te.G : 8)), L = tuned && te.L > 0 ? (size_t)te.L : 32;
+    const size_t G = plan_G((uint32_t)B, tuned ? (uint32_t)te.G : 8u), L = tuned && te.L > 0 ? (size_t)te.L : 32;
     const size_t E = n * W, nb = W * B, chunks = (E + L - 1) / L + 1;
-    size_t b = sizeof(hs::BatchDesc) + n * 64 + n + E * 2 + E * 4 + 4 + (nb + 1) * 4 + nb * sizeof(hs::xyzz) +
-               (nb / G) * sizeof(hs::xyzz) + W * sizeof(hs::xyzz) + sizeof(hs::jac) + chunks * 4 + 4 +
+    size_t b = sizeof(hs::BatchDesc) + n * hs::BASE_RECORD_BYTES + n + E * 2 + E * 4 + 4 + (nb + 1) * 4 + nb * sizeof(hs::xyzz) +
+               hs::seg_out_entries(W, (uint32_t)(B / G)) * sizeof(hs::xyzz) + W * sizeof(hs::xyzz) + sizeof(hs::jac) + chunks * 4 + 4 +
                2 * chunks * sizeof(hs::xyzz) + nb * 4 + 64;
     if (!device_inputs)
         b += n * (32 + sizeof(hs_gpu_jacobian));
@@ -2124,7 +2127,6 @@ static Plan noalloc_plan(size_t n)
     p.nb = (uint32_t)p.W * p.B;
     p.L = 32;
     p.Q = 1;
-    p.segs = p.nb / p.G;
     return p;
 }
 

@@ -36,8 +36,9 @@ struct SeleneC
 };
 
 // ---------------------------------------------------------------- memory helpers
-// Affine bases live in HBM as canonical little-endian words, x then y (64 bytes per point);
-// xyzz points (buckets, partial sums) as raw limbs, 4 x 9 words (144 bytes = 9 x 16 bytes).
+// Fixed-base tables hold affine points as canonical little-endian words, x then y (64 bytes per
+// entry); the MSM kernels' bases use the 128-byte records below; xyzz points (buckets, partial
+// sums) are raw limbs, 4 x 9 words (144 bytes = 9 x 16 bytes).
 __device__ __forceinline__ void load_aff(aff &a, const uint32_t *base, uint32_t idx)
 {
     const uint4 *p = reinterpret_cast<const uint4 *>(base + 16ull * idx);
@@ -58,6 +59,50 @@ template <class F> __device__ __forceinline__ void store_aff(uint32_t *base, uin
     p[1] = make_uint4(xw[4], xw[5], xw[6], xw[7]);
     p[2] = make_uint4(yw[0], yw[1], yw[2], yw[3]);
     p[3] = make_uint4(yw[4], yw[5], yw[6], yw[7]);
+}
+
+// Base records of the MSM kernels: 128 bytes per point, one L2 line, so a random gather moves as
+// many bytes as the 64-byte canonical form, which also arrives in one 128-byte request.  The
+// record holds the canonical affine point in field.h's 29-bit limbs and also -y, so the bucket walk
+// neither converts words to limbs nor negates y per entry:
+//   words 0..7   x limbs 0..7          words 16..23  (-y) limbs 0..7
+//   words 8..15  y limbs 0..7          words 24..27  x8, y8, x8, (-y)8      (28..31 zero)
+// A negative digit reads words 16..23 and 26..27 in place of 8..15 and 24..25.  The identity is
+// stored as zeros (its digits are all zero, so no entry reads it).
+constexpr uint32_t BASE_RECORD_WORDS = 32;
+constexpr size_t BASE_RECORD_BYTES = 4 * BASE_RECORD_WORDS;
+
+template <class F> __device__ __forceinline__ void store_base_record(uint32_t *base, uint32_t idx, const aff &a)
+{
+    uint32_t xw[8], yw[8], nw[8];
+    fe_to_words_canon<F>(xw, a.x);
+    fe_to_words_canon<F>(yw, a.y);
+    // -y = m - y, or 0 for y = 0
+    uint32_t yor = 0, br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+    {
+        const uint64_t t = (uint64_t)F::mw(i) - yw[i] - br;
+        nw[i] = (uint32_t)t;
+        br = (uint32_t)(t >> 32) & 1u;
+        yor |= yw[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        nw[i] = yor ? nw[i] : 0u;
+    fe x, y, ny;
+    fe_from_words(x, xw);
+    fe_from_words(y, yw);
+    fe_from_words(ny, nw);
+    uint4 *p = reinterpret_cast<uint4 *>(base + (size_t)BASE_RECORD_WORDS * idx);
+    p[0] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+    p[1] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
+    p[2] = make_uint4(y.v[0], y.v[1], y.v[2], y.v[3]);
+    p[3] = make_uint4(y.v[4], y.v[5], y.v[6], y.v[7]);
+    p[4] = make_uint4(ny.v[0], ny.v[1], ny.v[2], ny.v[3]);
+    p[5] = make_uint4(ny.v[4], ny.v[5], ny.v[6], ny.v[7]);
+    p[6] = make_uint4(x.v[8], y.v[8], x.v[8], ny.v[8]);
+    p[7] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __device__ __forceinline__ void store_xyzz(xyzz *dst, const xyzz &p)
@@ -102,7 +147,8 @@ __device__ __forceinline__ void load_xyzz(xyzz &p, const xyzz *src)
 
 // ---------------------------------------------------------------- ingest
 // Reference points: helios_jacobian = {X[5], Y[5], Z[5]} radix-2^51 uint64 limbs, Z = 0 is the
-// identity (helios/include/helios_ops.h:44-64).  Output canonical affine (X/Z^2, Y/Z^3).
+// identity (helios/include/helios_ops.h:44-64).  Output: the base record of the canonical affine
+// point (X/Z^2, Y/Z^3), store_base_record.
 // At most two waves per SIMD (the register allocation is padded to 169 VGPRs): with several MSMs
 // in flight, a full-occupancy ingest (107 VGPRs, four waves) crowds the CUs the other MSMs' bucket
 // kernels run on; four-stream throughput measured 650 M points/s against 720-730 M with the cap,
@@ -164,7 +210,7 @@ k_ingest(const uint64_t *__restrict__ jac, uint32_t n, uint32_t *__restrict__ ou
         }
         is_id[i] = 0;
     }
-    store_aff<F>(out, i, a);
+    store_base_record<F>(out, i, a);
 }
 
 // ---------------------------------------------------------------- exclusive scan (uint32)
@@ -386,34 +432,43 @@ __global__ void __launch_bounds__(SORT_FINE_THREADS) k_fine_sort(const uint32_t 
 // ---------------------------------------------------------------- bucket sums
 constexpr uint32_t ENTRY_SKIP = 0xffffffffu; // padding entry of the batched layout
 
-// Raw 64-byte base of an entry (point 0 for padding, which the walk then skips); issued one
-// entry ahead in the flat kernel so the gather overlaps the current addition.
-__device__ __forceinline__ void load_entry_raw(uint4 (&q)[4], const uint32_t *__restrict__ pts, uint32_t e)
+// The base of an entry as the bucket walk needs it: x and, by the entry's sign, y or -y (18
+// limbs) from its record, in flight one entry ahead so the gather overlaps the current addition.
+// Padding reads point 0, which the walk then skips.
+struct EntryBase
+{
+    uint4 x0, x1, y0, y1;
+    uint2 t; // limb 8 of x and of y or -y
+};
+
+__device__ __forceinline__ void load_entry_base(EntryBase &b, const uint32_t *__restrict__ pts, uint32_t e)
 {
     const uint32_t idx = e == ENTRY_SKIP ? 0u : (e & 0x7fffffffu);
-    const uint4 *p = reinterpret_cast<const uint4 *>(pts + 16ull * idx);
-    q[0] = p[0];
-    q[1] = p[1];
-    q[2] = p[2];
-    q[3] = p[3];
+    const uint32_t neg = e == ENTRY_SKIP ? 0u : e >> 31;
+    const uint32_t *r = pts + (size_t)BASE_RECORD_WORDS * idx;
+    b.x0 = *reinterpret_cast<const uint4 *>(r);
+    b.x1 = *reinterpret_cast<const uint4 *>(r + 4);
+    b.y0 = *reinterpret_cast<const uint4 *>(r + 8 + 8 * neg);
+    b.y1 = *reinterpret_cast<const uint4 *>(r + 12 + 8 * neg);
+    b.t = *reinterpret_cast<const uint2 *>(r + 24 + 2 * neg);
 }
 
-// acc += the entry's base (or its negation).  The caller passes no padding entry.
-template <class C>
-__device__ __forceinline__ void add_entry_raw(xyzz &acc, bool &acc_id, const uint4 (&q)[4], uint32_t e)
+__device__ __forceinline__ void entry_point(aff &p, const EntryBase &b)
 {
-    using F = typename C::F;
-    const uint32_t xw[8] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w};
-    const uint32_t yw[8] = {q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
+    p.x.v[0] = b.x0.x; p.x.v[1] = b.x0.y; p.x.v[2] = b.x0.z; p.x.v[3] = b.x0.w;
+    p.x.v[4] = b.x1.x; p.x.v[5] = b.x1.y; p.x.v[6] = b.x1.z; p.x.v[7] = b.x1.w;
+    p.x.v[8] = b.t.x;
+    p.y.v[0] = b.y0.x; p.y.v[1] = b.y0.y; p.y.v[2] = b.y0.z; p.y.v[3] = b.y0.w;
+    p.y.v[4] = b.y1.x; p.y.v[5] = b.y1.y; p.y.v[6] = b.y1.z; p.y.v[7] = b.y1.w;
+    p.y.v[8] = b.t.y;
+}
+
+// acc += the entry's base (its sign already applied by the load).  The caller passes no padding.
+template <class C> __device__ __forceinline__ void add_entry(xyzz &acc, bool &acc_id, const EntryBase &b)
+{
     aff p;
-    fe_from_words(p.x, xw);
-    fe_from_words(p.y, yw);
-    // a negative digit adds -P: y -> 65 m - y limb by limb (branch-free, lazy; xyzz_madd accepts it)
-    const uint32_t keep = (e >> 31) ? 0u : 0xffffffffu;
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        p.y.v[i] = (p.y.v[i] & keep) | ((F::bias2(i) - p.y.v[i]) & ~keep);
-    xyzz_madd_hot<F>(acc, acc_id, p);
+    entry_point(p, b);
+    xyzz_madd_hot<typename C::F>(acc, acc_id, p);
 }
 
 constexpr int BIG_THREADS = 256;
@@ -466,17 +521,13 @@ __global__ void __launch_bounds__(256) k_chunk_start(const uint32_t *__restrict_
         chunk_bucket[t] = g;
 }
 
-// A run of one bucket starts from its first entry's base, +-P as (x, y, 1, +-1)
-// (xyzz_from_aff_signed); a padding entry starts it as the identity.
-template <class F>
-__device__ __forceinline__ void run_start_raw(xyzz &acc, bool &acc_id, const uint4 (&q)[4], uint32_t e)
+// A run of one bucket starts from its first entry's base (sign applied); a padding entry starts
+// it as the identity.
+template <class F> __device__ __forceinline__ void run_start(xyzz &acc, bool &acc_id, const EntryBase &b, uint32_t e)
 {
-    const uint32_t xw[8] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w};
-    const uint32_t yw[8] = {q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
     aff p;
-    fe_from_words(p.x, xw);
-    fe_from_words(p.y, yw);
-    xyzz_from_aff_signed<F>(acc, p, (e >> 31) != 0);
+    entry_point(p, b);
+    xyzz_from_aff<F>(acc, p);
     acc_id = e == ENTRY_SKIP;
 }
 
@@ -492,8 +543,8 @@ struct FlatWalk
 // it.  The record of entry pos + 1 is issued into `nxt` first, so its gather overlaps the
 // addition.  Returns false once the chunk is done (the last run stored).
 template <class C>
-__device__ __forceinline__ bool flat_step(FlatWalk &f, xyzz &acc, bool &acc_id, uint32_t &e, const uint4 (&cur)[4],
-                                          uint4 (&nxt)[4], const uint32_t *__restrict__ offsets,
+__device__ __forceinline__ bool flat_step(FlatWalk &f, xyzz &acc, bool &acc_id, uint32_t &e, const EntryBase &cur,
+                                          EntryBase &nxt, const uint32_t *__restrict__ offsets,
                                           const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ pts,
                                           xyzz *__restrict__ buckets, xyzz *__restrict__ bnd)
 {
@@ -502,7 +553,7 @@ __device__ __forceinline__ bool flat_step(FlatWalk &f, xyzz &acc, bool &acc_id, 
     if (f.pos + 1 < f.end)
     {
         e = f.e_next;
-        load_entry_raw(nxt, pts, e);
+        load_entry_base(nxt, pts, e);
         if (f.pos + 2 < f.end)
             f.e_next = sorted[f.pos + 2];
     }
@@ -536,10 +587,10 @@ __device__ __forceinline__ bool flat_step(FlatWalk &f, xyzz &acc, bool &acc_id, 
             f.bucket_end = offsets[f.g + 1];
         }
         f.next_end = f.g + 2 <= f.g_hi ? offsets[f.g + 2] : f.end;
-        run_start_raw<F>(acc, acc_id, cur, e_cur);
+        run_start<F>(acc, acc_id, cur, e_cur);
     }
     else
-        add_entry_raw<C>(acc, acc_id, cur, e_cur);
+        add_entry<C>(acc, acc_id, cur);
     f.pos++;
     return true;
 }
@@ -567,20 +618,20 @@ __global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict_
     f.first = true;
     f.first_whole = offsets[f.g] == beg;
     // the first entry starts the first run; the walk then begins at beg + 1
-    uint4 qa[4], qb[4];
+    EntryBase qa, qb;
     uint32_t e = sorted[beg];
-    load_entry_raw(qa, pts, e);
+    load_entry_base(qa, pts, e);
     f.e_next = beg + 1 < f.end ? sorted[beg + 1] : ENTRY_SKIP;
     xyzz acc;
     bool acc_id;
-    run_start_raw<F>(acc, acc_id, qa, e);
+    run_start<F>(acc, acc_id, qa, e);
     f.pos = beg + 1;
     // entry beg + 1's record goes to qb; from then on the two buffers alternate (unrolled by two,
     // so the in-flight record never needs a register copy)
     if (f.pos < f.end)
     {
         e = f.e_next;
-        load_entry_raw(qb, pts, e);
+        load_entry_base(qb, pts, e);
         if (f.pos + 1 < f.end)
             f.e_next = sorted[f.pos + 1];
     }
@@ -659,30 +710,105 @@ __global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__re
 }
 
 // ---------------------------------------------------------------- bucket reduction
-// Window w, segment s covers buckets [s*G, s*G + G) (bucket b holds digit magnitude b + 1).
-// Running sums give T = sum (b - s*G + 1) B_b and R = sum B_b; the segment's share of
-// sum (b+1) B_b is T + (s*G) R (reference running-sum reduction, helios_msm_vartime.cpp:334-361).
-// The workgroup then adds its threads' shares over groups of tw = min(segs_per_win, 256)
-// consecutive segments (one window each) with an LDS tree: one partial sum per group, at index
-// (global segment index) / tw, so a window leaves segs_per_win / tw partials for k_reduce_win.
+// Window w's sum is sum_b (b + 1) B_b over its B buckets (reference running-sum reduction,
+// helios_msm_vartime.cpp:334-361).  Split the buckets into segments of G: segment s holds buckets
+// [s G, s G + G), and with R_s = sum_j B_(sG+j) and T_s = sum_j (j + 1) B_(sG+j) (a running sum,
+// one thread per segment)
+//     window sum = sum_s T_s + G sum_s s R_s.
+// sum_s s R_s = sum_i 2^i Y_i, where Y_i sums the R_s whose index s has bit i set.  These subset
+// sums come out of a binary tree over the segments that keeps, besides each node's total, the
+// totals of its right children at every level; no scalar multiple of any partial sum is formed,
+// and the only doublings left are those of the final sum_i 2^(i + log2 G) Y_i, one short chain
+// per window.
+//   k_reduce_seg: one thread per segment; a workgroup reduces tw = min(segments per window, 256)
+//                 consecutive segments of one window (256 / tw windows per workgroup) and leaves
+//                 2 + log2 tw points per block: S = sum R, T = sum T, Q_L = sum of the R_s with
+//                 bit L of s (within the block) set.
+//   k_reduce_win: one workgroup per window runs the same tree over the window's blocks (the
+//                 blocks' own Q_L add up; the block index's bits add subset sums of S), then its
+//                 first wavefront forms T + sum_i 2^(i + log2 G) Y_i with every field element
+//                 spread over a DPP row (rowcurve.h) and stores the window sum in Jacobian form.
+constexpr uint32_t SEG_SLOTS = 10;        // S, T and Q_0..Q_7 of a block of up to 256 segments
+constexpr uint32_t MAX_SEGS_PER_WIN = 8192; // k_reduce_win keeps (2 + 8) x 32 blocks in LDS
+
+__host__ __device__ inline uint32_t ilog2u(uint32_t x)
+{
+    uint32_t r = 0;
+    while (x >>= 1)
+        r++;
+    return r;
+}
+
+// segments per k_reduce_seg block
+__host__ __device__ inline uint32_t seg_block(uint32_t segs_per_win) { return segs_per_win < 256u ? segs_per_win : 256u; }
+
+// xyzz entries of k_reduce_seg's output for nwin windows
+__host__ __device__ inline size_t seg_out_entries(size_t nwin, uint32_t segs_per_win)
+{
+    return nwin * (segs_per_win / seg_block(segs_per_win)) * SEG_SLOTS;
+}
+
+// dynamic LDS of k_reduce_win
+__host__ __device__ inline size_t reduce_win_lds(uint32_t segs_per_win)
+{
+    const uint32_t tw = seg_block(segs_per_win), np = segs_per_win / tw, lt = ilog2u(tw), lp = ilog2u(np);
+    return (size_t)(2 + lt) * np * sizeof(xyzz) + (size_t)(lt + lp + 1) * sizeof(jac);
+}
+
+template <class F> __device__ __forceinline__ xyzz xyzz_sel(bool c, const xyzz &a, const xyzz &b)
+{
+    xyzz r;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        r.X.v[i] = c ? a.X.v[i] : b.X.v[i];
+        r.Y.v[i] = c ? a.Y.v[i] : b.Y.v[i];
+        r.ZZ.v[i] = c ? a.ZZ.v[i] : b.ZZ.v[i];
+        r.ZZZ.v[i] = c ? a.ZZZ.v[i] : b.ZZZ.v[i];
+    }
+    return r;
+}
+
+__device__ __forceinline__ xyzz xyzz_shfl_xor1(const xyzz &a)
+{
+    xyzz r;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+    {
+        r.X.v[i] = __shfl_xor(a.X.v[i], 1, 64);
+        r.Y.v[i] = __shfl_xor(a.Y.v[i], 1, 64);
+        r.ZZ.v[i] = __shfl_xor(a.ZZ.v[i], 1, 64);
+        r.ZZZ.v[i] = __shfl_xor(a.ZZZ.v[i], 1, 64);
+    }
+    return r;
+}
+
+// Thread t of the launch handles global segment w_lo * (B / G) + t (nseg segments in all).
+// Tree step s >= 1 combines, for each list, the entries at P and P + 2^s of one array (P = k 2^(s+1)
+// plus the list's offset): list 0 is S (offset 0), list 1 is T (stored at half the index, its
+// step 0 being done across neighbouring lanes), list 2 + L (L < s) the Q_L entries of S, which sit
+// at offset 2^L: the right children of level L, left untouched by the levels above.  Within a step
+// every write lands at an offset below 2^s (mod 2^(s+1)) and every partner read at one above, so
+// one barrier per step suffices; each thread does at most one addition per step.
 template <class C>
 __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buckets, const uint32_t *__restrict__ offsets,
                                                     uint32_t B, uint32_t G, uint32_t w_lo, uint32_t nseg,
                                                     xyzz *__restrict__ seg_out)
 {
     using F = typename C::F;
-    __shared__ xyzz sh[256];
-    const uint32_t segs_per_win = B / G;
-    const uint32_t t = w_lo * segs_per_win + blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = t < w_lo * segs_per_win + nseg;
-    xyzz T;
+    __shared__ xyzz shS[256];
+    __shared__ xyzz shT[128];
+    const uint32_t segs_per_win = B / G, tw = seg_block(segs_per_win), lt = ilog2u(tw);
+    const uint32_t t0 = w_lo * segs_per_win, i = threadIdx.x;
+    const uint32_t t = t0 + blockIdx.x * 256 + i;
+    const bool live = t < t0 + nseg;
+    xyzz R, T;
+    xyzz_identity<F>(R);
     xyzz_identity<F>(T);
     if (live)
     {
-        const uint32_t w = t / segs_per_win, s = t % segs_per_win;
-        const size_t g0 = (size_t)w * B + (size_t)s * G;
-        xyzz R;
-        xyzz_identity<F>(R);
+        const uint32_t w = t / segs_per_win, sg = t % segs_per_win;
+        const size_t g0 = (size_t)w * B + (size_t)sg * G;
         for (int j = (int)G - 1; j >= 0; j--)
         {
             if (offsets[g0 + j] != offsets[g0 + j + 1]) // empty buckets hold the identity
@@ -693,83 +819,133 @@ __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buc
             }
             xyzz_add<F>(T, T, R);
         }
-        xyzz U;
-        xyzz_mul_small<F>(U, R, s * G);
-        xyzz_add<F>(T, T, U);
     }
-    const uint32_t tw = segs_per_win < 256u ? segs_per_win : 256u, lt = threadIdx.x % tw;
-    store_xyzz(sh + threadIdx.x, T);
-    __syncthreads();
-    for (uint32_t h = tw >> 1; h > 0; h >>= 1)
+    if (lt == 0)
     {
-        if (lt < h)
+        // one segment per window: the block is the segment itself
+        if (live)
         {
-            xyzz x, y;
-            load_xyzz(x, sh + threadIdx.x);
-            load_xyzz(y, sh + threadIdx.x + h);
-            xyzz_add<F>(x, x, y);
-            store_xyzz(sh + threadIdx.x, x);
+            store_xyzz(seg_out + (size_t)t * SEG_SLOTS, R);
+            store_xyzz(seg_out + (size_t)t * SEG_SLOTS + 1, T);
+        }
+        return;
+    }
+    // step 0 across lanes 2k, 2k + 1: the even lane forms S = R_2k + R_2k+1, the odd lane
+    // T_2k + T_2k+1 and keeps R_2k+1 (the Q_0 entry) in place
+    const bool odd = (i & 1u) != 0;
+    const xyzz other = xyzz_shfl_xor1(xyzz_sel<F>(odd, R, T));
+    xyzz sum = xyzz_sel<F>(odd, T, R);
+    xyzz_add<F>(sum, sum, other);
+    store_xyzz(shS + i, xyzz_sel<F>(odd, R, sum));
+    if (odd)
+        store_xyzz(shT + (i >> 1), sum);
+    __syncthreads();
+    for (uint32_t s = 1; s < lt; s++)
+    {
+        const uint32_t nk = 128u >> s;
+        if (i < (2 + s) * nk)
+        {
+            const uint32_t lst = i / nk, P = (i % nk) << (s + 1);
+            xyzz *dst;
+            uint32_t h;
+            if (lst == 1)
+            {
+                dst = shT + (P >> 1);
+                h = 1u << (s - 1);
+            }
+            else
+            {
+                dst = shS + (lst == 0 ? P : P + (1u << (lst - 2)));
+                h = 1u << s;
+            }
+            xyzz a, b;
+            load_xyzz(a, dst);
+            load_xyzz(b, dst + h);
+            xyzz_add<F>(a, a, b);
+            store_xyzz(dst, a);
         }
         __syncthreads();
     }
-    if (lt == 0 && live)
+    const uint32_t nslot = 2 + lt, nblk = 256 / tw;
+    for (uint32_t x = i; x < nblk * nslot; x += 256)
     {
-        load_xyzz(T, sh + threadIdx.x);
-        store_xyzz(seg_out + t / tw, T);
+        const uint32_t j = x / nslot, slot = x % nslot, base = blockIdx.x * 256 + j * tw;
+        if (base >= nseg)
+            continue;
+        const xyzz *src = slot == 0 ? shS + j * tw : slot == 1 ? shT + (j * tw >> 1) : shS + j * tw + (1u << (slot - 2));
+        xyzz v;
+        load_xyzz(v, src);
+        store_xyzz(seg_out + (size_t)((t0 + base) / tw) * SEG_SLOTS + slot, v);
     }
 }
 
-// partials per window that k_reduce_seg leaves
-__host__ __device__ inline uint32_t seg_partials(uint32_t segs_per_win) { return segs_per_win < 256u ? 1u : segs_per_win / 256u; }
-
-// Sums of the windows' segment results.  A window of segs_per_win segments (a power of two) gets
-// tw = min(segs_per_win, 256) threads, so one workgroup covers 256 / tw windows: each thread adds
-// a strided share of its window's segments, then an LDS tree over the tw threads of the window.
-// (Batches of small MSMs have windows of only a few segments; one workgroup each left almost all
-// of its threads summing identities.)
+// One workgroup (256 threads) per window w_lo + blockIdx.x; dynamic LDS reduce_win_lds(B / G).
 template <class C>
-__global__ void __launch_bounds__(BIG_THREADS) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win,
-                                                            uint32_t w_lo, uint32_t nwin, jac *__restrict__ win_out)
+__global__ void __launch_bounds__(256) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win, uint32_t G,
+                                                    uint32_t w_lo, jac *__restrict__ win_out)
 {
     using F = typename C::F;
-    __shared__ xyzz sh[BIG_THREADS];
-    const uint32_t tw = segs_per_win < (uint32_t)BIG_THREADS ? segs_per_win : (uint32_t)BIG_THREADS;
-    const uint32_t per_blk = BIG_THREADS / tw, lw = threadIdx.x / tw, lt = threadIdx.x % tw;
-    const uint32_t wl = blockIdx.x * per_blk + lw; // window index within the launch
-    const bool live = wl < nwin;
-    const uint32_t w = w_lo + wl;
-    xyzz acc;
-    xyzz_identity<F>(acc);
-    if (live)
-        for (uint32_t s = lt; s < segs_per_win; s += tw)
-        {
-            xyzz b;
-            load_xyzz(b, seg_out + (size_t)w * segs_per_win + s);
-            xyzz_add<F>(acc, acc, b);
-        }
-    store_xyzz(sh + threadIdx.x, acc);
-    __syncthreads();
-    for (uint32_t h = tw >> 1; h > 0; h >>= 1)
+    extern __shared__ uint4 lds_win[];
+    const uint32_t tw = seg_block(segs_per_win), lt = ilog2u(tw), np = segs_per_win / tw, lp = ilog2u(np);
+    const uint32_t nslot = 2 + lt, nY = lt + lp, i = threadIdx.x, w = w_lo + blockIdx.x;
+    xyzz *A = reinterpret_cast<xyzz *>(lds_win); // slot-major: A[slot * np + block]
+    jac *J = reinterpret_cast<jac *>(A + (size_t)nslot * np);
+    for (uint32_t x = i; x < nslot * np; x += 256)
     {
-        if (lt < h)
+        const uint32_t k = x / nslot, slot = x % nslot;
+        xyzz v;
+        load_xyzz(v, seg_out + ((size_t)w * np + k) * SEG_SLOTS + slot);
+        store_xyzz(A + (size_t)slot * np + k, v);
+    }
+    __syncthreads();
+    // the tree over the window's blocks: every slot array adds up, and the bits of the block index
+    // give new subset sums of S at offsets 2^L
+    for (uint32_t s = 0; s < lp; s++)
+    {
+        const uint32_t nk = np >> (s + 1);
+        for (uint32_t x = i; x < (nslot + s) * nk; x += 256)
         {
-            xyzz x, y;
-            load_xyzz(x, sh + threadIdx.x);
-            load_xyzz(y, sh + threadIdx.x + h);
-            xyzz_add<F>(x, x, y);
-            store_xyzz(sh + threadIdx.x, x);
+            const uint32_t lst = x / nk, P = (x % nk) << (s + 1);
+            xyzz *dst = lst < nslot ? A + (size_t)lst * np + P : A + P + (1u << (lst - nslot));
+            xyzz a, b;
+            load_xyzz(a, dst);
+            load_xyzz(b, dst + (1u << s));
+            xyzz_add<F>(a, a, b);
+            store_xyzz(dst, a);
         }
         __syncthreads();
     }
-    if (lt == 0 && live)
+    // Y_i (i < lt): the blocks' Q_i summed; Y_(lt + L): S subset sums; J[nY] = T
+    if (i <= nY)
     {
-        // the window sum leaves in Jacobian form, converted here in parallel over windows rather
-        // than inside the serial Horner chain
-        load_xyzz(acc, sh + threadIdx.x);
-        jac J;
-        xyzz_to_jac<F>(J, acc);
-        win_out[w] = J;
+        const xyzz *src = i == nY ? A + np : i < lt ? A + (size_t)(2 + i) * np : A + (1u << (i - lt));
+        xyzz v;
+        load_xyzz(v, src);
+        jac r;
+        xyzz_to_jac<F>(r, v);
+        J[i] = r;
     }
+    __syncthreads();
+    if (i >= 64)
+        return;
+    // window sum = T + 2^g sum_i 2^i Y_i (Horner on one wavefront, row-distributed)
+    row::RowK<F> k;
+    row::rowk_init(k);
+    row::rjac acc, tt;
+    if (nY == 0)
+        row::rjac_load<F>(k, acc, J);
+    else
+    {
+        row::rhorner<F>(k, acc, J, 0, (int)nY, 1, true, nullptr);
+        if (!row::ris_zero<F>(acc.Z))
+            row::rjac_dbl_n<F>(k, acc, (int)ilog2u(G));
+        row::rjac_load<F>(k, tt, J + nY);
+        row::rjac_add<F>(k, acc, acc, tt);
+    }
+    jac r;
+    row::rjac_get<F>(r, acc);
+    if (i == 0)
+        win_out[w] = r;
 }
 
 // ---------------------------------------------------------------- window combine + encode
@@ -1136,7 +1312,7 @@ __global__ void __launch_bounds__(256) k_ingest_batch(const BatchDesc *__restric
         }
         is_id[i] = 0;
     }
-    store_aff<F>(out, i, a);
+    store_base_record<F>(out, i, a);
 }
 
 template <class C>

@@ -315,9 +315,9 @@ template <class F> static void madd_chain(uint8_t *out, const uint8_t *pts, cons
     out[64] = ok;
 }
 
-// The bucket walk's form of the same sum: the first non-identity point starts the run as
-// (x, y, 1, +-1) (xyzz_from_aff_signed, y not negated), the rest go through xyzz_madd_hot with a
-// lazily negated y, as k_bucket_flat does.
+// The bucket walk's form of the same sum: each point arrives with its sign applied (y or a
+// normalised -y, as the base records store them), the first non-identity point starts the run
+// (xyzz_from_aff) and the rest go through xyzz_madd_hot, as k_bucket_flat does.
 template <class F> static void walk_chain(uint8_t *out, const uint8_t *pts, const uint8_t *neg, size_t n)
 {
     xyzz acc;
@@ -333,15 +333,15 @@ template <class F> static void walk_chain(uint8_t *out, const uint8_t *pts, cons
         aff q;
         ld(q.x, pts + 64 * i);
         ld(q.y, pts + 64 * i + 32);
+        if (neg[i])
+            fe_neg<F>(q.y, q.y);
         if (!started)
         {
-            xyzz_from_aff_signed<F>(acc, q, neg[i] != 0);
+            xyzz_from_aff<F>(acc, q);
             aid = false;
             started = true;
             continue;
         }
-        if (neg[i])
-            fe_neg_lazy2<F>(q.y, q.y);
         xyzz_madd_hot<F>(acc, aid, q);
     }
     if (aid)

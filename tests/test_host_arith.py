@@ -123,9 +123,9 @@ def test_point_ops_vs_oracle(L, c):
 @pytest.mark.parametrize("c", [0, 1])
 def test_madd_chain_vs_oracle(L, c, fn):
     """Long xyzz_madd chains with repeated points, negated points, identities and cancellations,
-    against the oracle's sum.  hst_walk_chain is the bucket walk's form: the run starts from its
-    first point as (x, y, 1, +-1) and continues with xyzz_madd_hot; a negated first point (trial 2)
-    exercises the ZZZ = -1 start."""
+    against the oracle's sum.  hst_walk_chain is the bucket walk's form: points arrive with their
+    sign applied, the run starts from its first point and continues with xyzz_madd_hot (trial 2
+    starts with a negated point, trial 3 cancels the first point with the second)."""
     rnd = random.Random(40 + c)
     g = O.generator(c)
     base = [O.scalar_mul(c, rnd.getrandbits(250).to_bytes(32, "little"), g) for _ in range(40)]
