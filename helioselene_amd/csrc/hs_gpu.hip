@@ -479,8 +479,6 @@ int scan_exclusive(Workspace &cx, const uint32_t *in, uint32_t *out, uint32_t le
     return 0;
 }
 
-// windows per k_reduce_win workgroup (256 threads, min(segments, 256) of them per window)
-
 // The plain MSM on device-resident inputs.  d_scalars: n x 8 words; d_points: n x 15 uint64.
 // Point indices in the sort entries have 24 bits: larger MSMs run as chunks (msm_device_any).
 constexpr size_t MSM_MAX_CHUNK = (size_t)1 << 23;
@@ -633,7 +631,7 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         const uint32_t nseg = (uint32_t)(w_hi - w_lo) * segs_per_win;
         hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((nseg + 255) / 256), blk, 0, st, buckets, offsets, P.B, P.G,
                            (uint32_t)w_lo, nseg, seg);
-        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((uint32_t)(w_hi - w_lo)), dim3(256), hs::reduce_win_lds(segs_per_win),
+        hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3((uint32_t)(w_hi - w_lo)), dim3(64), hs::reduce_win_lds(segs_per_win),
                            st, seg, segs_per_win, P.G, (uint32_t)w_lo, win);
         cx.mark(st, 4);
         if (Q > 1)
@@ -749,7 +747,7 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
                        g.bnd.as<hs::xyzz>(), g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>(), buckets);
     hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((g.nseg + 255) / 256), blk, 0, st, buckets, offsets, g.B, g.G, 0u, g.nseg,
                        seg);
-    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(g.V), dim3(256), hs::reduce_win_lds(g.B / g.G), st, seg, g.B / g.G, g.G,
+    hipLaunchKernelGGL(hs::k_reduce_win<C>, dim3(g.V), dim3(64), hs::reduce_win_lds(g.B / g.G), st, seg, g.B / g.G, g.G,
                        0u, win);
     if (g_horner_quad)
     {

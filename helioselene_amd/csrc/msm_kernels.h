@@ -724,10 +724,10 @@ __global__ void __launch_bounds__(BIG_THREADS) k_merge_long(const uint32_t *__re
 //                 consecutive segments of one window (256 / tw windows per workgroup) and leaves
 //                 2 + log2 tw points per block: S = sum R, T = sum T, Q_L = sum of the R_s with
 //                 bit L of s (within the block) set.
-//   k_reduce_win: one workgroup per window runs the same tree over the window's blocks (the
-//                 blocks' own Q_L add up; the block index's bits add subset sums of S), then its
-//                 first wavefront forms T + sum_i 2^(i + log2 G) Y_i with every field element
-//                 spread over a DPP row (rowcurve.h) and stores the window sum in Jacobian form.
+//   k_reduce_win: one wavefront per window runs the same tree over the window's blocks (the
+//                 blocks' own Q_L add up; the block index's bits add subset sums of S), then
+//                 forms T + sum_i 2^(i + log2 G) Y_i with every field element spread over a DPP
+//                 row (rowcurve.h) and stores the window sum in Jacobian form.
 constexpr uint32_t SEG_SLOTS = 10;        // S, T and Q_0..Q_7 of a block of up to 256 segments
 constexpr uint32_t MAX_SEGS_PER_WIN = 8192; // k_reduce_win keeps (2 + 8) x 32 blocks in LDS
 
@@ -879,9 +879,10 @@ __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buc
     }
 }
 
-// One workgroup (256 threads) per window w_lo + blockIdx.x; dynamic LDS reduce_win_lds(B / G).
+// One wavefront per window w_lo + blockIdx.x (64-thread workgroups: batches have thousands of
+// small windows, whose workgroups must not hold idle waves); dynamic LDS reduce_win_lds(B / G).
 template <class C>
-__global__ void __launch_bounds__(256) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win, uint32_t G,
+__global__ void __launch_bounds__(64) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win, uint32_t G,
                                                     uint32_t w_lo, jac *__restrict__ win_out)
 {
     using F = typename C::F;
@@ -890,7 +891,7 @@ __global__ void __launch_bounds__(256) k_reduce_win(const xyzz *__restrict__ seg
     const uint32_t nslot = 2 + lt, nY = lt + lp, i = threadIdx.x, w = w_lo + blockIdx.x;
     xyzz *A = reinterpret_cast<xyzz *>(lds_win); // slot-major: A[slot * np + block]
     jac *J = reinterpret_cast<jac *>(A + (size_t)nslot * np);
-    for (uint32_t x = i; x < nslot * np; x += 256)
+    for (uint32_t x = i; x < nslot * np; x += 64)
     {
         const uint32_t k = x / nslot, slot = x % nslot;
         xyzz v;
@@ -903,7 +904,7 @@ __global__ void __launch_bounds__(256) k_reduce_win(const xyzz *__restrict__ seg
     for (uint32_t s = 0; s < lp; s++)
     {
         const uint32_t nk = np >> (s + 1);
-        for (uint32_t x = i; x < (nslot + s) * nk; x += 256)
+        for (uint32_t x = i; x < (nslot + s) * nk; x += 64)
         {
             const uint32_t lst = x / nk, P = (x % nk) << (s + 1);
             xyzz *dst = lst < nslot ? A + (size_t)lst * np + P : A + P + (1u << (lst - nslot));
@@ -926,8 +927,6 @@ __global__ void __launch_bounds__(256) k_reduce_win(const xyzz *__restrict__ seg
         J[i] = r;
     }
     __syncthreads();
-    if (i >= 64)
-        return;
     // window sum = T + 2^g sum_i 2^i Y_i (Horner on one wavefront, row-distributed)
     row::RowK<F> k;
     row::rowk_init(k);

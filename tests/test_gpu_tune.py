@@ -10,9 +10,13 @@ hs = pytest.importorskip("helioselene_amd")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("params", [(9, 16, 4), (11, 64, 16), (13, 7, 2), (5, 1, 1)])
+# (14, 32, 1) and (16, 32, 1) give the window reduction its largest shape, 8192 segments in 32
+# blocks per window (the plan raises the latter's G to 4, so that no window has more segments);
+# (5, 1, 1) and (11, 64, 16) give windows of one block, (13, 7, 2) eight blocks.
+@pytest.mark.parametrize("params", [(9, 16, 4), (11, 64, 16), (13, 7, 2), (5, 1, 1), (14, 32, 1), (16, 32, 1)])
 def test_forced_parameters_keep_parity(tmp_path, params):
     c, L, G = params
+    plan_G = max(G, (1 << (c - 1)) // 8192)
     n = 1500  # also in the reference's window-8 band: compat substitution must still apply
     f = tmp_path / "t.txt"
     f.write_text("".join(f"{m} {cv} 10 {c} {L} {G} 1.0\n" for m in (0, 1) for cv in (0, 1)))
@@ -20,7 +24,7 @@ def test_forced_parameters_keep_parity(tmp_path, params):
     hs.tune_load(str(f))
     try:
         for cv in (0, 1):
-            assert hs.tune_get(cv, 0, n) == (c, L, G)
+            assert hs.tune_get(cv, 0, n) == (c, L, plan_G)
             sc = O.gen_scalars(cv, 900 + c, n, edge_frac=0.02)
             pb = O.gen_sswu_bases(cv, 910 + c, n, dup_frac=0.02, id_frac=0.01)
             pts = GH.to_jac_array(cv, pb, n)
