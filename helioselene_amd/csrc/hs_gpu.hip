@@ -1597,6 +1597,39 @@ struct SrvLane
     bool busy = false;
 };
 
+// Request latencies of a server: totals over every completion since the last reset, and the most
+// recent SERVER_LAT_WINDOW values for the percentiles (a ring, so a long-lived server keeps a
+// bounded log).
+constexpr size_t SERVER_LAT_WINDOW = (size_t)1 << 20;
+
+struct LatencyLog
+{
+    std::vector<double> ring;
+    size_t next = 0;
+    uint64_t count = 0;
+    double sum = 0, max = 0;
+    void add(double us)
+    {
+        if (ring.size() < SERVER_LAT_WINDOW)
+            ring.push_back(us);
+        else
+        {
+            ring[next] = us;
+            next = (next + 1) % SERVER_LAT_WINDOW;
+        }
+        count++;
+        sum += us;
+        max = std::max(max, us);
+    }
+    void clear()
+    {
+        ring.clear();
+        next = 0;
+        count = 0;
+        sum = max = 0;
+    }
+};
+
 struct hs_gpu_server
 {
     size_t lane_bytes = 0;
@@ -1610,7 +1643,7 @@ struct hs_gpu_server
     uint64_t next_id = 1;
     bool stop = false;
     std::thread th;
-    std::vector<double> lat_us;
+    LatencyLog lat;
     uint64_t points_done = 0, batches = 0;
     std::chrono::steady_clock::time_point t0;
 };
@@ -1653,7 +1686,7 @@ void srv_finish(hs_gpu_server &s, SrvLane &ln, int status)
                 std::memcpy(q->d.out32, ln.pin_out + 32 * i, 32);
             if (q->d.result)
                 std::memcpy(q->d.result, ln.pin_out + 32 * k + sizeof(hs_gpu_jacobian) * i, sizeof(hs_gpu_jacobian));
-            s.lat_us.push_back(std::chrono::duration<double, std::micro>(now - q->t_submit).count());
+            s.lat.add(std::chrono::duration<double, std::micro>(now - q->t_submit).count());
             s.points_done += q->d.n;
         }
         q->t_done = now;
@@ -2003,19 +2036,16 @@ int hs_gpu_server_get_stats(hs_gpu_server *s, hs_gpu_server_stats *out)
     if (!s || !out)
         return fail(HS_GPU_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(s->mu);
-    std::vector<double> v = s->lat_us;
+    std::vector<double> v = s->lat.ring;
     std::sort(v.begin(), v.end());
     auto pct = [&v](double p) { return v.empty() ? 0.0 : v[std::min(v.size() - 1, (size_t)(p * (v.size() - 1) + 0.5))]; };
-    out->completed = v.size();
+    out->completed = s->lat.count;
     out->batches = s->batches;
     out->points = s->points_done;
     out->p50_us = pct(0.50);
     out->p99_us = pct(0.99);
-    out->max_us = v.empty() ? 0.0 : v.back();
-    double sum = 0;
-    for (double x : v)
-        sum += x;
-    out->mean_us = v.empty() ? 0.0 : sum / v.size();
+    out->max_us = s->lat.max;
+    out->mean_us = s->lat.count ? s->lat.sum / (double)s->lat.count : 0.0;
     out->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
     return 0;
 }
@@ -2025,7 +2055,7 @@ int hs_gpu_server_reset_stats(hs_gpu_server *s)
     if (!s)
         return fail(HS_GPU_EINVAL, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
-    s->lat_us.clear();
+    s->lat.clear();
     s->points_done = 0;
     s->batches = 0;
     s->t0 = std::chrono::steady_clock::now();

@@ -231,7 +231,9 @@ void hs_gpu_fixed_destroy(hs_gpu_fixed *set);
  * flags semantics).
  * hs_gpu_server_wait returns the request's status (0 or HS_GPU_E*), or 1 if timeout_ms (>= 0)
  * expired first; a completed ticket is released by the wait that returns its status.
- * Latencies are submit-to-completion, measured on the host. */
+ * Latencies are submit-to-completion, measured on the host.  hs_gpu_server_get_stats counts,
+ * averages and takes the maximum over every completion since the server started or its stats were
+ * reset; p50 and p99 are taken over the most recent 2^20 completions (the server keeps no more). */
 typedef struct hs_gpu_server_config
 {
     size_t device_bytes; /* HBM for in-flight requests, all lanes together; 0 = 8 GiB */
