@@ -708,15 +708,17 @@ def main():
     tuning = None
     if not args.no_tune:
         # the library's autotuner picks (window bits, run length, segment size) per size class on
-        # this device: plain MSMs at the headline size, batch groups over the batched / served /
-        # Pedersen size range
+        # this device: plain MSMs from 2^6 up to the headline size (the mix's per-MSM launches use the
+        # small classes), batch groups over the batched / served / Pedersen size range
         t_tune = time.perf_counter()
         lg = max(6, min(22, n.bit_length() - 1))
         for c in (0, 1):
-            hs.autotune(c, 0, lg, lg)
+            hs.autotune(c, 0, 6, lg)
             hs.autotune(c, 1, 6, 18)
         tuning = {"seconds": time.perf_counter() - t_tune,
                   "plain": {cv: hs.tune_get(c, 0, n) for c, cv in enumerate(("helios", "selene"))},
+                  "plain_by_size": {cv: {f"2^{k}": hs.tune_get(c, 0, 1 << k) for k in range(6, lg + 1)}
+                                    for c, cv in enumerate(("helios", "selene"))},
                   "batch": {cv: {f"2^{k}": hs.tune_get(c, 1, 1 << k) for k in range(6, 19)}
                             for c, cv in enumerate(("helios", "selene"))},
                   "fields": "(window bits c, entries per thread L, buckets per segment G)"}

@@ -748,11 +748,32 @@ __host__ __device__ inline size_t seg_out_entries(size_t nwin, uint32_t segs_per
     return nwin * (segs_per_win / seg_block(segs_per_win)) * SEG_SLOTS;
 }
 
-// dynamic LDS of k_reduce_win
-__host__ __device__ inline size_t reduce_win_lds(uint32_t segs_per_win)
+// LDS of one window's wavefront in k_reduce_win
+__host__ __device__ inline size_t reduce_win_lds1(uint32_t segs_per_win)
 {
     const uint32_t tw = seg_block(segs_per_win), np = segs_per_win / tw, lt = ilog2u(tw), lp = ilog2u(np);
     return (size_t)(2 + lt) * np * sizeof(xyzz) + (size_t)(lt + lp + 1) * sizeof(jac);
+}
+
+// dynamic LDS of k_reduce_win with wg windows per workgroup: each wavefront's own area, then one
+// Jacobian exchange slot per wavefront
+__host__ __device__ inline size_t reduce_win_lds(uint32_t segs_per_win, uint32_t wg = 1)
+{
+    return ((reduce_win_lds1(segs_per_win) + 15) & ~(size_t)15) * wg + (size_t)wg * sizeof(jac);
+}
+
+// Windows per group of the window combine: a workgroup of wg wavefronts reduces wg consecutive
+// windows at once and adds them up as sum_j 2^(c j) S_(wg k + j), so the serial Horner over the
+// groups has W / wg additions (each about two doublings' worth) in place of W, for (wg - 1) c
+// doublings and log2 wg additions done beside it.  Worth it when additions are many and doublings
+// per window few: 4 windows per group up to c = 8 (W >= 32), 2 up to c = 12, else 1.  Capped so
+// the workgroup's LDS stays within 48 KiB.
+__host__ __device__ inline uint32_t win_group(int c, uint32_t segs_per_win)
+{
+    uint32_t wg = c <= 8 ? 4u : c <= 12 ? 2u : 1u;
+    while (wg > 1 && reduce_win_lds(segs_per_win, wg) > 48u * 1024u)
+        wg >>= 1;
+    return wg;
 }
 
 template <class F> __device__ __forceinline__ xyzz xyzz_sel(bool c, const xyzz &a, const xyzz &b)
@@ -879,45 +900,57 @@ __global__ void __launch_bounds__(256) k_reduce_seg(const xyzz *__restrict__ buc
     }
 }
 
-// One wavefront per window w_lo + blockIdx.x (64-thread workgroups: batches have thousands of
-// small windows, whose workgroups must not hold idle waves); dynamic LDS reduce_win_lds(B / G).
+// Workgroups of wg wavefronts (64 threads each; batches have thousands of small windows, whose
+// workgroups must not hold idle waves).  Workgroup b handles group gi = b % groups of chain
+// ch = b / groups: wavefront j reduces window wc = gi wg + j of the chain (global window
+// w_lo + ch Wc + wc; a wavefront past the chain's Wc windows only joins the barriers), scales its
+// sum by 2^(c j) and the group's sums are added by an LDS tree; the result, in Jacobian form,
+// goes to win_out[w_lo + ch groups + gi].  Dynamic LDS: reduce_win_lds(segs_per_win, wg).
 template <class C>
-__global__ void __launch_bounds__(64) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win, uint32_t G,
-                                                    uint32_t w_lo, jac *__restrict__ win_out)
+__global__ void __launch_bounds__(256) k_reduce_win(const xyzz *__restrict__ seg_out, uint32_t segs_per_win, uint32_t G,
+                                                     uint32_t w_lo, uint32_t Wc, uint32_t groups, int c,
+                                                     jac *__restrict__ win_out)
 {
     using F = typename C::F;
     extern __shared__ uint4 lds_win[];
     const uint32_t tw = seg_block(segs_per_win), lt = ilog2u(tw), np = segs_per_win / tw, lp = ilog2u(np);
-    const uint32_t nslot = 2 + lt, nY = lt + lp, i = threadIdx.x, w = w_lo + blockIdx.x;
-    xyzz *A = reinterpret_cast<xyzz *>(lds_win); // slot-major: A[slot * np + block]
+    const uint32_t nslot = 2 + lt, nY = lt + lp, i = threadIdx.x & 63u, j = threadIdx.x >> 6, wg = blockDim.x >> 6;
+    const uint32_t ch = blockIdx.x / groups, gi = blockIdx.x % groups, wc = gi * wg + j;
+    const bool live = wc < Wc;
+    const uint32_t w = w_lo + ch * Wc + wc;
+    const size_t area = (reduce_win_lds1(segs_per_win) + 15) & ~(size_t)15;
+    xyzz *A = reinterpret_cast<xyzz *>(reinterpret_cast<char *>(lds_win) + area * j); // slot-major: A[slot * np + block]
     jac *J = reinterpret_cast<jac *>(A + (size_t)nslot * np);
-    for (uint32_t x = i; x < nslot * np; x += 64)
-    {
-        const uint32_t k = x / nslot, slot = x % nslot;
-        xyzz v;
-        load_xyzz(v, seg_out + ((size_t)w * np + k) * SEG_SLOTS + slot);
-        store_xyzz(A + (size_t)slot * np + k, v);
-    }
+    jac *X = reinterpret_cast<jac *>(reinterpret_cast<char *>(lds_win) + area * wg); // exchange slots
+    if (live)
+        for (uint32_t x = i; x < nslot * np; x += 64)
+        {
+            const uint32_t k = x / nslot, slot = x % nslot;
+            xyzz v;
+            load_xyzz(v, seg_out + ((size_t)w * np + k) * SEG_SLOTS + slot);
+            store_xyzz(A + (size_t)slot * np + k, v);
+        }
     __syncthreads();
     // the tree over the window's blocks: every slot array adds up, and the bits of the block index
     // give new subset sums of S at offsets 2^L
     for (uint32_t s = 0; s < lp; s++)
     {
         const uint32_t nk = np >> (s + 1);
-        for (uint32_t x = i; x < (nslot + s) * nk; x += 64)
-        {
-            const uint32_t lst = x / nk, P = (x % nk) << (s + 1);
-            xyzz *dst = lst < nslot ? A + (size_t)lst * np + P : A + P + (1u << (lst - nslot));
-            xyzz a, b;
-            load_xyzz(a, dst);
-            load_xyzz(b, dst + (1u << s));
-            xyzz_add<F>(a, a, b);
-            store_xyzz(dst, a);
-        }
+        if (live)
+            for (uint32_t x = i; x < (nslot + s) * nk; x += 64)
+            {
+                const uint32_t lst = x / nk, P = (x % nk) << (s + 1);
+                xyzz *dst = lst < nslot ? A + (size_t)lst * np + P : A + P + (1u << (lst - nslot));
+                xyzz a, b;
+                load_xyzz(a, dst);
+                load_xyzz(b, dst + (1u << s));
+                xyzz_add<F>(a, a, b);
+                store_xyzz(dst, a);
+            }
         __syncthreads();
     }
     // Y_i (i < lt): the blocks' Q_i summed; Y_(lt + L): S subset sums; J[nY] = T
-    if (i <= nY)
+    if (live && i <= nY)
     {
         const xyzz *src = i == nY ? A + np : i < lt ? A + (size_t)(2 + i) * np : A + (1u << (i - lt));
         xyzz v;
@@ -927,24 +960,50 @@ __global__ void __launch_bounds__(64) k_reduce_win(const xyzz *__restrict__ seg_
         J[i] = r;
     }
     __syncthreads();
-    // window sum = T + 2^g sum_i 2^i Y_i (Horner on one wavefront, row-distributed)
+    // window sum = T + 2^g sum_i 2^i Y_i (Horner on one wavefront, row-distributed), then 2^(c j)
+    // times it
     row::RowK<F> k;
     row::rowk_init(k);
     row::rjac acc, tt;
-    if (nY == 0)
-        row::rjac_load<F>(k, acc, J);
+    if (!live)
+        row::rjac_identity<F>(k, acc);
     else
     {
-        row::rhorner<F>(k, acc, J, 0, (int)nY, 1, true, nullptr);
-        if (!row::ris_zero<F>(acc.Z))
-            row::rjac_dbl_n<F>(k, acc, (int)ilog2u(G));
-        row::rjac_load<F>(k, tt, J + nY);
-        row::rjac_add<F>(k, acc, acc, tt);
+        if (nY == 0)
+            row::rjac_load<F>(k, acc, J);
+        else
+        {
+            row::rhorner<F>(k, acc, J, 0, (int)nY, 1, true, nullptr);
+            if (!row::ris_zero<F>(acc.Z))
+                row::rjac_dbl_n<F>(k, acc, (int)ilog2u(G));
+            row::rjac_load<F>(k, tt, J + nY);
+            row::rjac_add<F>(k, acc, acc, tt);
+        }
+        if (j > 0 && !row::ris_zero<F>(acc.Z))
+            row::rjac_dbl_n<F>(k, acc, c * (int)j);
     }
+    // the group's sums added up by a tree over the wavefronts (every branch below is wave-uniform)
     jac r;
+    for (uint32_t s = 1; s < wg; s <<= 1)
+    {
+        if ((j & (2 * s - 1)) == s)
+        {
+            row::rjac_get<F>(r, acc);
+            if (i == 0)
+                X[j] = r;
+        }
+        __syncthreads();
+        if ((j & (2 * s - 1)) == 0 && j + s < wg)
+        {
+            row::rjac_load<F>(k, tt, X + j + s);
+            row::rjac_add<F>(k, acc, acc, tt);
+        }
+    }
+    if (j != 0)
+        return;
     row::rjac_get<F>(r, acc);
     if (i == 0)
-        win_out[w] = r;
+        win_out[w_lo + ch * groups + gi] = r;
 }
 
 // ---------------------------------------------------------------- window combine + encode
