@@ -631,8 +631,8 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
                                    ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
         }
         cx.mark(st, 3);
-        hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.bnd.as<hs::xyzz>(), 8u,
-                           buckets, big_list, big_count);
+        hipLaunchKernelGGL(hs::k_merge_span<C>, dim3((uint32_t)((chunks + 255) / 256)), blk, 0, st, offsets, g_lo, g_hi, L,
+                           ws.chunk_bucket.as<uint32_t>(), ws.bnd.as<hs::xyzz>(), 8u, buckets, big_list, big_count);
         hipLaunchKernelGGL(hs::k_merge_long<C>, dim3(256), dim3(hs::BIG_THREADS), 0, st, offsets, g_lo, L,
                            ws.bnd.as<hs::xyzz>(), big_list, big_count, buckets);
         cx.mark(st, 6);
@@ -755,8 +755,9 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
     hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, 0u, g.nb, g.L, g.chunk_bucket.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((g.chunks + 255) / 256)), blk, 0, st, offsets, sorted, 0u,
                        g.nb, g.L, g.chunk_bucket.as<uint32_t>(), aff, buckets, g.bnd.as<hs::xyzz>());
-    hipLaunchKernelGGL(hs::k_merge_span<C>, gnb, blk, 0, st, offsets, 0u, g.nb, g.L, g.bnd.as<hs::xyzz>(), 8u, buckets,
-                       g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>());
+    hipLaunchKernelGGL(hs::k_merge_span<C>, dim3((uint32_t)((g.chunks + 255) / 256)), blk, 0, st, offsets, 0u, g.nb, g.L,
+                       g.chunk_bucket.as<uint32_t>(), g.bnd.as<hs::xyzz>(), 8u, buckets, g.long_list.as<uint32_t>(),
+                       g.long_count.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_merge_long<C>, dim3(128), dim3(hs::BIG_THREADS), 0, st, offsets, 0u, g.L,
                        g.bnd.as<hs::xyzz>(), g.long_list.as<uint32_t>(), g.long_count.as<uint32_t>(), buckets);
     hipLaunchKernelGGL(hs::k_reduce_seg<C>, dim3((g.nseg + 255) / 256), blk, 0, st, buckets, offsets, g.B, g.G, 0u, g.nseg,

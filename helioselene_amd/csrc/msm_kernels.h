@@ -645,24 +645,29 @@ __global__ void __launch_bounds__(256) k_bucket_flat(const uint32_t *__restrict_
 }
 
 // Completes buckets whose entries span several chunks; spans longer than `short_max` chunks
-// are queued for k_merge_long.
+// are queued for k_merge_long.  One thread per chunk t (entries base + [t L, t L + L), base =
+// offsets[g_lo], nchunks = ceil((offsets[g_hi] - base) / L)): the thread of the chunk in which a
+// cut bucket ends adds it up, so every cut bucket has exactly one thread and there are about
+// half as many threads as buckets (a bucket per thread left most lanes idle).
 template <class C>
 __global__ void __launch_bounds__(256) k_merge_span(const uint32_t *__restrict__ offsets, uint32_t g_lo, uint32_t g_hi,
-                                                    uint32_t L, const xyzz *__restrict__ bnd, uint32_t short_max,
+                                                    uint32_t L, const uint32_t *__restrict__ chunk_bucket,
+                                                    const xyzz *__restrict__ bnd, uint32_t short_max,
                                                     xyzz *__restrict__ buckets, uint32_t *__restrict__ long_list,
                                                     uint32_t *__restrict__ long_count)
 {
     using F = typename C::F;
-    const uint32_t g = g_lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= g_hi)
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t base = offsets[g_lo], E = offsets[g_hi] - base;
+    if ((uint64_t)t * L >= E)
         return;
-    const uint32_t base = offsets[g_lo];
+    // the bucket holding the chunk's first entry: cut here if it started in an earlier chunk, and
+    // completed here if it ends in this chunk
+    const uint32_t g = chunk_bucket[t];
     const uint32_t a = offsets[g] - base, e = offsets[g + 1] - base;
-    if (a == e)
+    if (a == t * L || (uint64_t)e > (uint64_t)t * L + L)
         return;
-    const uint32_t t0 = a / L, t1 = (e - 1) / L;
-    if (t0 == t1)
-        return;
+    const uint32_t t0 = a / L, t1 = t;
     if (t1 - t0 + 1 > short_max)
     {
         long_list[atomicAdd(long_count, 1u)] = g;
@@ -670,9 +675,9 @@ __global__ void __launch_bounds__(256) k_merge_span(const uint32_t *__restrict__
     }
     xyzz acc, p;
     load_xyzz(acc, bnd + 2 * (size_t)t0 + (a % L ? 1 : 0));
-    for (uint32_t t = t0 + 1; t <= t1; t++)
+    for (uint32_t u = t0 + 1; u <= t1; u++)
     {
-        load_xyzz(p, bnd + 2 * (size_t)t);
+        load_xyzz(p, bnd + 2 * (size_t)u);
         xyzz_add<F>(acc, acc, p);
     }
     store_xyzz(buckets + g, acc);
