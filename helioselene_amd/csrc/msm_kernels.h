@@ -768,15 +768,18 @@ __host__ __device__ inline size_t reduce_win_lds(uint32_t segs_per_win, uint32_t
 }
 
 // Windows per group of the window combine: a workgroup of wg wavefronts reduces wg consecutive
-// windows at once and adds them up as sum_j 2^(c j) S_(wg k + j), so the serial Horner over the
-// groups has W / wg additions (each about two doublings' worth) in place of W, for (wg - 1) c
-// doublings and log2 wg additions done beside it.  Worth it when additions are many and doublings
-// per window few: 4 windows per group up to c = 8 (W >= 32), 2 up to c = 12, else 1.  Capped so
-// the workgroup's LDS stays within 48 KiB.
+// windows at once and adds them up as sum_j 2^(c j) S_(wg k + j).  The serial Horner over the
+// groups then has ceil(W / wg) - 1 additions in place of W - 1 (a Jacobian addition costs about
+// two doublings); its doublings, c (W - wg), plus the c (wg - 1) of the longest scaling beside it
+// are the c (W - 1) of the plain Horner, and the group's own log2 wg additions run on the side.
+// Four windows per group up to c = 13 (W >= 20) where the workgroup's LDS fits in 64 KiB, else
+// two, else one.  At c = 16 (W = 16) two windows per group measured no faster on MI355X: the
+// Horner's 7 fewer additions (-20 us Helios, -37 us Selene) went into the longer window
+// reduction (+19 / +34 us).
 __host__ __device__ inline uint32_t win_group(int c, uint32_t segs_per_win)
 {
-    uint32_t wg = c <= 8 ? 4u : c <= 12 ? 2u : 1u;
-    while (wg > 1 && reduce_win_lds(segs_per_win, wg) > 48u * 1024u)
+    uint32_t wg = c <= 13 ? 4u : 1u;
+    while (wg > 1 && reduce_win_lds(segs_per_win, wg) > 64u * 1024u)
         wg >>= 1;
     return wg;
 }
