@@ -37,6 +37,31 @@ def test_forced_parameters_keep_parity(tmp_path, params):
         hs.tune_clear()
 
 
+# Every window width, plain and batched: the window sums are combined in groups of up to four
+# windows (W = ceil(256 / c) windows; c = 6, 7, 9, 10 and 12 leave a ragged last group, c >= 14 one
+# window per group), each group's sums scaled by 2^(c j) and added before the Horner over groups.
+@pytest.mark.parametrize("c", list(range(4, 17)))
+def test_every_window_width_keeps_parity(tmp_path, c):
+    n = 700
+    f = tmp_path / "t.txt"
+    f.write_text("".join(f"{m} {cv} 9 {c} 16 4 1.0\n" for m in (0, 1) for cv in (0, 1)))
+    hs.tune_clear()
+    hs.tune_load(str(f))
+    try:
+        for cv in (0, 1):
+            assert hs.tune_get(cv, 0, n)[0] == c
+            sc = O.gen_scalars(cv, 300 + c, n, edge_frac=0.02)
+            pb = O.gen_sswu_bases(cv, 310 + c, n, dup_frac=0.02, id_frac=0.01)
+            pts = GH.to_jac_array(cv, pb, n)
+            want = O.msm(cv, sc, pb, n)
+            assert hs.msm(cv, GH.scalars_np(sc, n), pts)[0] == want
+            got = hs.msm_batch([(cv, GH.scalars_np(sc, n), pts), (1 - cv, np.zeros((0, 32), np.uint8),
+                                                                  np.zeros((0, 15), np.uint64))])
+            assert got[0][0] == want
+    finally:
+        hs.tune_clear()
+
+
 def test_autotune_installs_parameters(tmp_path):
     f = tmp_path / "tune.txt"
     hs.tune_clear()
