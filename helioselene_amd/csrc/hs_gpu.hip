@@ -162,8 +162,13 @@ int choose_c(size_t n)
 // reduction segment) of a plain MSM.  Below 2^15 points the MSM is a chain of latency-bound
 // kernels: every bucket-kernel thread adds its L entries one after another and every reduction
 // thread 2 G buckets, so short runs and segments finish first (measured on MI355X: 0.65 -> 0.56 ms
-// per MSM over sizes 64 .. 2^14).
-uint32_t plain_default_L(size_t n) { return n < ((size_t)1 << 15) ? 16u : 32u; }
+// per MSM over sizes 64 .. 2^14).  From 2^19 points, runs of 64 entries cut half as many buckets
+// as runs of 32 (buckets average 32 entries at 2^20), which halves the merge: whole MSM at 2^20
+// 1.61 / 2.27 ms against 1.66 / 2.33 ms (Helios / Selene).
+uint32_t plain_default_L(size_t n)
+{
+    return n < ((size_t)1 << 15) ? 16u : n < ((size_t)1 << 19) ? 32u : 64u;
+}
 uint32_t plain_default_G(size_t n) { return n < ((size_t)1 << 15) ? 4u : 8u; }
 
 // Tuning overrides for experiments: HS_GPU_C (window bits), HS_GPU_G (buckets per reduction

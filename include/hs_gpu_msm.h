@@ -264,7 +264,8 @@ void hs_gpu_server_destroy(hs_gpu_server *server); /* drains queued and running 
  * running-sum segment).  mode 0 tunes the plain MSM (latency of one MSM); mode 1 tunes a group of
  * a batch (time of many MSMs of that size in one batch).  It searches coordinate-wise (c, then L,
  * then G) from the built-in defaults (plain MSMs: c = floor(log2 n) - 4 clamped to [4, 16];
- * L = 16 and G = 4 below 2^15 points, L = 32 and G = 8 above), over L in {8, 16, 32, 64} and G in
+ * L = 16 and G = 4 below 2^15 points, L = 32 and G = 8 up to 2^19, L = 64 and G = 8 from 2^19),
+ * over L in {8, 16, 32, 64} and G in
  * {2, 4, 8, 16}, keeping a candidate only when it is more than 3% faster
  * (best of three timed runs), installs the result into the library's table (used by all
  * later plans, batches and server admissions of that size class +-1) and appends lines

@@ -26,7 +26,8 @@ def test_tune_file_lookup(lib_or_skip, tmp_path):
     assert hs.tune_get(0, 0, 1 << 16) == (11, 64, 4)
     assert hs.tune_get(0, 0, (1 << 17) - 1) == (11, 64, 4)   # same class
     assert hs.tune_get(0, 0, 1 << 17) == (11, 64, 4)         # neighbouring class
-    assert hs.tune_get(0, 0, 1 << 19) == (15, 32, 8)         # too far: default
+    assert hs.tune_get(0, 0, 1 << 19) == (15, 64, 8)         # too far: default (runs of 64 from 2^19)
+    assert hs.tune_get(0, 0, 1 << 10) == (6, 16, 4)          # default below 2^15: short runs and segments
     assert hs.tune_get(1, 0, 1 << 16) == (12, 32, 8)         # other curve: default
     assert hs.tune_get(1, 1, 1 << 10) == (7, 16, 16)
     assert hs.tune_get(0, 1, 1 << 10) == (6, 32, 8)          # batch default for the other curve
