@@ -695,6 +695,38 @@ int choose_c_batch(size_t n)
     return c > 13 ? 13 : c;
 }
 
+// Plan parameters (c, L, G) of a batch member.  A member that is its curve's only MSM in the batch
+// forms a group of its own, whose time is the latency of one MSM: it takes the plain MSM's
+// parameters (tuning mode 0, latency of one MSM), with c capped at 13.  Other members take the
+// batch parameters (mode 1, the time of many MSMs of that size in one batch).
+struct MemberPlan
+{
+    int c;
+    uint32_t L, G;
+};
+
+MemberPlan member_plan(int curve, size_t n, bool alone)
+{
+    TuneEntry te;
+    MemberPlan m;
+    if (alone)
+    {
+        const bool tuned = tune_lookup(0, curve, n, te);
+        m.c = tuned ? te.c : choose_c(n);
+        m.L = tuned && te.L > 0 ? (uint32_t)te.L : plain_default_L(n);
+        m.G = tuned ? (uint32_t)te.G : plain_default_G(n);
+    }
+    else
+    {
+        const bool tuned = tune_lookup(1, curve, n, te);
+        m.c = tuned ? te.c : choose_c_batch(n);
+        m.L = tuned && te.L > 0 ? (uint32_t)te.L : 32u;
+        m.G = tuned ? (uint32_t)te.G : 8u;
+    }
+    m.c = m.c < 4 ? 4 : m.c > 13 ? 13 : m.c;
+    return m;
+}
+
 // Bump allocator over one device block (the serving loop's per-lane memory).
 struct Arena
 {
@@ -837,15 +869,20 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
     // are merged into at most HS_GPU_BATCH_GROUPS (default 2) groups per curve: each width maps
     // to the largest width of its share of the curve's sorted distinct widths.  Batches planned
     // inside a serving arena keep their own widths (the admission footprint assumes them).
+    size_t per_curve[2] = {0, 0};
+    for (size_t i = 0; i < count; i++)
+        if (descs[i].n != 0 && descs[i].n < (1u << 24) &&
+            (descs[i].curve == HS_GPU_HELIOS || descs[i].curve == HS_GPU_SELENE))
+            per_curve[descs[i].curve]++;
     std::vector<int> want_c(count, 0);
+    std::vector<MemberPlan> mp(count);
     for (size_t i = 0; i < count; i++)
     {
         if (descs[i].n == 0 || descs[i].n >= (1u << 24) ||
             (descs[i].curve != HS_GPU_HELIOS && descs[i].curve != HS_GPU_SELENE))
             continue;
-        TuneEntry te;
-        const int c = tune_lookup(1, descs[i].curve, descs[i].n, te) ? te.c : choose_c_batch(descs[i].n);
-        want_c[i] = c < 4 ? 4 : c > 13 ? 13 : c;
+        mp[i] = member_plan(descs[i].curve, descs[i].n, per_curve[descs[i].curve] == 1);
+        want_c[i] = mp[i].c;
     }
     const int max_groups = arena ? 64 : std::max(1, env_int("HS_GPU_BATCH_GROUPS", 2));
     int cmap[2][14] = {};
@@ -890,8 +927,6 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
             hs_gpu_batch_destroy(b);
             return fail(HS_GPU_EINVAL, "batch element " + std::to_string(i) + ": n >= 2^24 (the plain MSM takes larger sizes)");
         }
-        TuneEntry te;
-        const bool tuned = tune_lookup(1, d.curve, d.n, te);
         const int c = cmap[d.curve][want_c[i]];
         const size_t Wc = (256 + c - 1) / c, Bc = (size_t)1 << (c - 1);
         // A group's entry slots (N * W), bucket ids (M * W * B) and point indices are 32-bit values
@@ -921,9 +956,8 @@ int batch_create(hs_gpu_batch **out, const hs_gpu_msm_desc *descs, size_t count,
         // the group runs with the largest run length and segment size of its members: its buffers
         // (chunk bounds, segment sums) are then no larger than the members' footprints together,
         // which is what the serving loop admits by
-        const uint32_t Gi = tuned ? (uint32_t)te.G : 8u, Li = tuned && te.L > 0 ? (uint32_t)te.L : 32u;
-        g->G = std::max(g->G, Gi);
-        g->L = std::max(g->L, Li);
+        g->G = std::max(g->G, mp[i].G);
+        g->L = std::max(g->L, mp[i].L);
         hs::BatchDesc bd;
         bd.scalars = (const uint32_t *)d.scalars;
         bd.points = (const uint64_t *)d.points;
@@ -1679,16 +1713,20 @@ size_t request_footprint(int curve, size_t n, bool device_inputs)
 {
     if (n == 0)
         return 1024;
-    TuneEntry te;
-    const bool tuned = tune_lookup(1, curve, n, te);
-    int c = tuned ? te.c : choose_c_batch(n);
-    c = c < 4 ? 4 : c > 13 ? 13 : c;
-    const size_t W = (256 + c - 1) / c, B = (size_t)1 << (c - 1);
-    const size_t G = plan_G((uint32_t)B, tuned ? (uint32_t)te.G : 8u), L = tuned && te.L > 0 ? (size_t)te.L : 32;
-    const size_t E = n * W, nb = W * B, chunks = (E + L - 1) / L + 1;
-    size_t b = sizeof(hs::BatchDesc) + n * hs::BASE_RECORD_BYTES + n + E * 2 + E * 4 + 4 + (nb + 1) * 4 + nb * sizeof(hs::xyzz) +
-               hs::seg_out_entries(W, (uint32_t)(B / G)) * sizeof(hs::xyzz) + W * sizeof(hs::xyzz) + sizeof(hs::jac) + chunks * 4 + 4 +
-               2 * chunks * sizeof(hs::xyzz) + nb * 4 + 64;
+    // the request's batch may hold it alone (plain parameters) or with others (batch parameters):
+    // admit by the larger footprint
+    size_t b = 0;
+    for (const bool alone : {false, true})
+    {
+        const MemberPlan m = member_plan(curve, n, alone);
+        const size_t W = (256 + m.c - 1) / m.c, B = (size_t)1 << (m.c - 1);
+        const size_t G = plan_G((uint32_t)B, m.G), L = std::max<size_t>(1, m.L);
+        const size_t E = n * W, nb = W * B, chunks = (E + L - 1) / L + 1;
+        b = std::max(b, sizeof(hs::BatchDesc) + n * hs::BASE_RECORD_BYTES + n + E * 2 + E * 4 + 4 + (nb + 1) * 4 +
+                            nb * sizeof(hs::xyzz) + hs::seg_out_entries(W, (uint32_t)(B / G)) * sizeof(hs::xyzz) +
+                            W * sizeof(hs::xyzz) + sizeof(hs::jac) + chunks * 4 + 4 + 2 * chunks * sizeof(hs::xyzz) + nb * 4 +
+                            64);
+    }
     if (!device_inputs)
         b += n * (32 + sizeof(hs_gpu_jacobian));
     return b + 32 + sizeof(hs_gpu_jacobian) + 16 * 256;
