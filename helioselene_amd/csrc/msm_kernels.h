@@ -323,16 +323,31 @@ __global__ void __launch_bounds__(SORT_COARSE_THREADS) k_recode_count(
     for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
         hist[j] = 0;
     __syncthreads();
+    // the next point's scalar and identity flag are loaded while this point is recoded
+    uint4 a = make_uint4(0, 0, 0, 0), bb = a;
+    uint8_t idf = 0;
+    if (beg + threadIdx.x < end)
+    {
+        const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * (beg + threadIdx.x));
+        a = sp[0];
+        bb = sp[1];
+        idf = is_id[beg + threadIdx.x];
+    }
     for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x)
     {
         uint32_t s[8];
-        const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * i);
-        const uint4 a = sp[0], bb = sp[1];
         s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
         s[4] = bb.x; s[5] = bb.y; s[6] = bb.z; s[7] = bb.w;
+        const bool skip = idf != 0;
+        if (i + blockDim.x < end)
+        {
+            const uint4 *sp = reinterpret_cast<const uint4 *>(scalars + 8ull * (i + blockDim.x));
+            a = sp[0];
+            bb = sp[1];
+            idf = is_id[i + blockDim.x];
+        }
         uint32_t k[8];
         scalar_prepare<typename C::O>(k, s, w8_compat != 0);
-        const bool skip = is_id[i] != 0;
         uint32_t carry = 0;
         for (int w = 0; w < W; w++)
         {
