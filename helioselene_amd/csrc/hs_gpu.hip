@@ -695,10 +695,13 @@ int choose_c_batch(size_t n)
     return c > 13 ? 13 : c;
 }
 
-// Plan parameters (c, L, G) of a batch member.  A member that is its curve's only MSM in the batch
-// forms a group of its own, whose time is the latency of one MSM: it takes the plain MSM's
-// parameters (tuning mode 0, latency of one MSM), with c capped at 13.  Other members take the
-// batch parameters (mode 1, the time of many MSMs of that size in one batch).
+// Plan parameters (c, L, G) of a batch member.  A member of fewer than 2^15 points that is its
+// curve's only MSM in the batch forms a group of its own, whose time is the latency of one MSM: it
+// takes the plain MSM's parameters (tuning mode 0, latency of one MSM, measured on an idle device),
+// with c capped at 13.  Other members take the batch parameters (mode 1, the time of many MSMs of
+// that size in one batch).  Larger lone members keep the batch parameters: served alone next to
+// other lanes' work, plain parameters made the server's largest requests slower (p99 over sizes up
+// to 2^18: 3.03-3.10 ms with them, 2.05-2.20 ms without, same p50).
 struct MemberPlan
 {
     int c;
@@ -709,7 +712,7 @@ MemberPlan member_plan(int curve, size_t n, bool alone)
 {
     TuneEntry te;
     MemberPlan m;
-    if (alone)
+    if (alone && n < ((size_t)1 << 15))
     {
         const bool tuned = tune_lookup(0, curve, n, te);
         m.c = tuned ? te.c : choose_c(n);
