@@ -819,14 +819,20 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
     return 0;
 }
 
-// Enqueues every group of the batch: fork from st onto the worker streams, join back into st.
-// Works eagerly and under stream capture alike.
+// Enqueues every group of the batch: fork from st onto the worker streams, join back into st (a
+// single group runs on st).  Works eagerly and under stream capture alike.
 int batch_enqueue(hs_gpu_batch &b, hipStream_t st)
 {
     const size_t K = b.workers.size();
     if (b.n_empty)
         hipLaunchKernelGGL(hs::k_identity_out, dim3((b.n_empty + 63) / 64), dim3(64), 0, st,
                            b.empty.as<hs::BatchDesc>(), b.n_empty);
+    // one group has nothing to overlap with: it runs on st itself, without the fork and join
+    if (b.groups.size() == 1)
+    {
+        BatchGroup &g = *b.groups[0];
+        return g.curve == HS_GPU_HELIOS ? batch_group_enqueue<hs::HeliosC>(g, st) : batch_group_enqueue<hs::SeleneC>(g, st);
+    }
     HS_CHECK(hipEventRecord(b.ev[0], st));
     for (size_t k = 0; k < K; k++)
         HS_CHECK(hipStreamWaitEvent(b.workers[k], b.ev[0], 0));
