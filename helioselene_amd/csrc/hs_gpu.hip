@@ -597,7 +597,6 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
     hipLaunchKernelGGL(hs::k_recode_count<C>, dim3(nblk), dim3(hs::SORT_COARSE_THREADS), lds, st, d_scalars, isid, nn,
                        P.c, P.W, compat ? 1 : 0, nsb, shift, T, nblk, digits, cnt_a);
     cx.mark(st, 1);
-    hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, cnt_a + (size_t)nh * nblk, 1u, 0u);
     if ((r = scan_exclusive(ws, cnt_a, off_a, geo.scan_len, st)))
         return r;
     hipLaunchKernelGGL(hs::k_coarse_scatter, dim3(nblk), dim3(hs::SORT_COARSE_THREADS), lds, st, digits, nn, P.W, nsb, shift, T, nblk, off_a,
@@ -617,8 +616,8 @@ int msm_device(Ctx &cx, const uint32_t *d_scalars, const uint64_t *d_points, siz
         const uint32_t g_lo = (uint32_t)w_lo * P.B, g_hi = (uint32_t)w_hi * P.B, ng = g_hi - g_lo;
         const dim3 gnb((ng + 255) / 256);
         const size_t chunks = ((size_t)n * (w_hi - w_lo) + L - 1) / L;
-        hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, big_count, 1u, 0u);
-        hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>());
+        hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>(),
+                           big_count);
         if (P.affK == 0)
             hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((chunks + 255) / 256)), blk, 0, st, offsets, sorted,
                                g_lo, g_hi, L, ws.chunk_bucket.as<uint32_t>(), aff, buckets, ws.bnd.as<hs::xyzz>());
@@ -785,14 +784,14 @@ template <class C> int batch_group_enqueue(BatchGroup &g, hipStream_t st)
     uint32_t *aff = g.aff.as<uint32_t>(), *sorted = g.sorted.as<uint32_t>(), *offsets = g.offsets.as<uint32_t>();
     hs::xyzz *buckets = g.buckets.as<hs::xyzz>(), *seg = g.seg.as<hs::xyzz>();
     hs::jac *win = g.win.as<hs::jac>(); // window sums, Jacobian
-    hipLaunchKernelGGL(hs::k_fill_u32, dim3(1), dim3(64), 0, st, g.long_count.as<uint32_t>(), 1u, 0u);
     hipLaunchKernelGGL(hs::k_ingest_batch<C>, dim3((N + 255) / 256), blk, 0, st, d, g.M, N, aff, g.isid.as<uint8_t>());
     hipLaunchKernelGGL(hs::k_recode_batch<C>, dim3((N + 255) / 256), blk, 0, st, d, g.M, N, g.isid.as<uint8_t>(), g.c, g.W,
                        g.digits.as<uint16_t>());
     hipLaunchKernelGGL(hs::k_window_sort, dim3(g.V), blk, 0, st, d, N, g.W, g.B, g.digits.as<uint16_t>(), sorted, offsets,
                        g.V);
     const dim3 gnb((g.nb + 255) / 256);
-    hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, 0u, g.nb, g.L, g.chunk_bucket.as<uint32_t>());
+    hipLaunchKernelGGL(hs::k_chunk_start, gnb, blk, 0, st, offsets, 0u, g.nb, g.L, g.chunk_bucket.as<uint32_t>(),
+                       g.long_count.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_bucket_flat<C>, dim3((uint32_t)((g.chunks + 255) / 256)), blk, 0, st, offsets, sorted, 0u,
                        g.nb, g.L, g.chunk_bucket.as<uint32_t>(), aff, buckets, g.bnd.as<hs::xyzz>());
     hipLaunchKernelGGL(hs::k_merge_span<C>, dim3((uint32_t)((g.chunks + 255) / 256)), blk, 0, st, offsets, 0u, g.nb, g.L,

@@ -322,6 +322,8 @@ __global__ void __launch_bounds__(SORT_COARSE_THREADS) k_recode_count(
     const uint32_t b = blockIdx.x, beg = b * T, end = min(beg + T, n), nh = (uint32_t)W * nsb;
     for (uint32_t j = threadIdx.x; j < nh; j += blockDim.x)
         hist[j] = 0;
+    if (b == 0 && threadIdx.x == 0)
+        counts[(size_t)nh * nblk] = 0; // the element past the last count: the scan's total
     __syncthreads();
     // the next point's scalar and identity flag are loaded while this point is recoded
     uint4 a = make_uint4(0, 0, 0, 0), bb = a;
@@ -523,10 +525,14 @@ template <class C> __device__ void block_tree_sum(xyzz &acc, xyzz *sh)
 
 // The kernels below work on the buckets [g_lo, g_hi) of a window range; chunk t covers the entries
 // base + [t*L, t*L + L), base = offsets[g_lo].
-// chunk_bucket[t] = the bucket holding entry base + t*L
+// chunk_bucket[t] = the bucket holding entry base + t*L; the first thread also clears *zero_word
+// (the merge's count of long spans) when given
 __global__ void __launch_bounds__(256) k_chunk_start(const uint32_t *__restrict__ offsets, uint32_t g_lo, uint32_t g_hi,
-                                                     uint32_t L, uint32_t *__restrict__ chunk_bucket)
+                                                     uint32_t L, uint32_t *__restrict__ chunk_bucket,
+                                                     uint32_t *__restrict__ zero_word)
 {
+    if (zero_word && blockIdx.x == 0 && threadIdx.x == 0)
+        *zero_word = 0;
     uint32_t g = g_lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= g_hi)
         return;
