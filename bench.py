@@ -30,6 +30,8 @@ The same line carries:
   * `roofline.hbm_phases`: achieved HBM bandwidth of the sort/scatter and gather phases;
   * `config1_helios_4096`: Helios n = 4096 on k*G points, GPU and reference CPU;
   * `fcmpp_verify`: the FCMP++ verification MSM pairs (Z != 1 bases), GPU and reference CPU;
+  * `fcmpp_tree_hash`: FCMP++ tree-hash layers (Selene 228 / 38, Helios 18 fixed generators), as a
+    batch and over a resident fixed-base set, against the reference per node;
   * `cpu_baseline`: the reference's own CPU MSM (compiled from its sources into
     oracle/_ref/libhsref.so) on a bounded sample of the same inputs, one pinned thread, median
     of three runs, with the CPU model.
@@ -330,6 +332,79 @@ def run_fcmpp(hs, torch, args, st, L):
             tc = sum(cpu_time_median(L, c, sc, pts) for c, sc, pts in host)
             rec.update({"cpu_ms": tc * 1e3, "cpu_points_per_s": npts / tc})
         out["uniform_scalars" if uniform else "batches"][str(b)] = rec
+    return out
+
+
+def run_tree_hash(hs, torch, args, st, L):
+    """FCMP++ tree hashes (src/benchmark_fcmpp.cpp:133-198 of the reference): a node's hash is an
+    MSM over its layer's fixed generators (doubling chains, Z != 1) plus the affine conversion
+    (the 32-byte result carries the canonical x).  Per layer shape (Selene 228 / 38, Helios 18
+    children): one layer of nodes with uniform 255-bit scalars as one batch (captured graph) and over
+    a resident fixed-base set of the generators (16-tooth comb), a single node's latency, and the
+    reference's MSM + tobytes per node on one host thread."""
+    import numpy as np
+
+    from helioselene_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    ref = {(r["curve"], r["n"], r["index"]): r["ref_result"]
+           for r in (json.load(open(os.path.join(ROOT, "tests", "golden", "workload_runs.json"))))
+           if r["kind"] == "tree_hash"} if os.path.exists(os.path.join(ROOT, "tests", "golden", "workload_runs.json")) else {}
+    out = {"workload": "one tree layer per shape: nodes of n children over the layer's doubling-chain generators, "
+                       "uniform 255-bit scalars; a node = MSM + affine conversion (32-byte result)", "layers": {}}
+    sh = st.cuda_stream
+
+    def timed(fn, reps):
+        lat = []
+        with torch.cuda.stream(st):
+            for i in range(2 + reps):
+                t0 = time.perf_counter()
+                fn()
+                st.synchronize()
+                lat.append(time.perf_counter() - t0)
+        return float(np.median(lat[2:]))
+
+    for c, n, name in W.TREE_HASH_SHAPES:
+        K = 2048 if n > 100 else 4096
+        pts = W.doubling_chain(c, n)
+        sc = W.tree_layer_scalars(n, K, 7000 + n)
+        with torch.cuda.stream(st):
+            dp = torch.from_numpy(pts.view(np.int64)).to(dev)
+            ds = torch.from_numpy(sc).to(dev)
+            o_b = torch.zeros((K, 8), dtype=torch.int32, device=dev)
+            o_f = torch.zeros((K, 8), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        items = [(c, n, ds.data_ptr() + 32 * n * h, dp.data_ptr(), o_b[h].data_ptr(), None) for h in range(K)]
+        bt = hs.Batch(items)
+        bt.capture()
+        t_batch = timed(lambda: bt.launch(sh), 5)
+        bt.close()
+        t_single = timed(lambda: hs.msm_device(c, items[0][2], items[0][3], n, items[0][4], None, 0, sh), 10)
+        t0 = time.perf_counter()
+        fs = hs.FixedSet(c, d_points=dp.data_ptr(), n=n, stream=sh)
+        torch.cuda.synchronize()
+        t_build = time.perf_counter() - t0
+        t_fixed = timed(lambda: fs.msm_device(ds.data_ptr(), K, o_f.data_ptr(), method=hs.FIXED_COMB, stream=sh), 5)
+        fs.close()
+        got_b = [o_b[h].cpu().numpy().tobytes().hex() for h in range(4)]
+        got_f = [o_f[h].cpu().numpy().tobytes().hex() for h in range(4)]
+        want = [ref.get((("helios", "selene")[c], n, h)) for h in range(4)]
+        rec = {"curve": ("helios", "selene")[c], "children": n, "nodes": K,
+               "gpu_single_node_ms": t_single * 1e3,
+               "gpu_batch_ms": t_batch * 1e3, "gpu_batch_nodes_per_s": K / t_batch,
+               "gpu_fixed_comb_ms": t_fixed * 1e3, "gpu_fixed_comb_nodes_per_s": K / t_fixed,
+               "fixed_table_build_ms": t_build * 1e3,
+               "bit_exact_vs_reference": (got_b == want and got_f == want) if None not in want else None,
+               "batch_equals_fixed": bool((o_b.cpu().numpy() == o_f.cpu().numpy()).all())}
+        if L is not None:
+            m = 256 if n < 100 else 64
+            t0 = time.perf_counter()
+            for h in range(m):
+                cpu_ref_bytes(L, c, sc[h], pts)
+            tc = (time.perf_counter() - t0) / m
+            rec.update({"cpu_node_ms": tc * 1e3, "cpu_nodes_per_s": 1.0 / tc,
+                        "cpu_sample": f"{m} nodes, reference MSM + tobytes, one host thread"})
+        out["layers"][name] = rec
     return out
 
 
@@ -874,6 +949,7 @@ def main():
     L = None if args.no_cpu else ref_lib()
     rec["config1_helios_4096"] = None if args.no_config1 else run_config1(hs, torch, args, streams[0], L)
     rec["fcmpp_verify"] = None if args.no_fcmpp else run_fcmpp(hs, torch, args, streams[0], L)
+    rec["fcmpp_tree_hash"] = None if args.no_fcmpp else run_tree_hash(hs, torch, args, streams[0], L)
     rec["batched_mix"] = None if args.no_mix else run_mix(hs, torch, args)
     rec["pedersen_batch"] = None if args.no_pedersen else run_pedersen(hs, torch, args)
     rec["serving"] = None if args.no_serve else run_serving(hs, torch, args)

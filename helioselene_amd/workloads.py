@@ -120,6 +120,21 @@ def fcmpp_scalars(n):
     return sc
 
 
+# FCMP++ tree hashes (src/benchmark_fcmpp.cpp:133-198): a node's hash is an MSM over its layer's
+# fixed generators (the benchmark's doubling chains) followed by to_affine; the layers hash 228
+# (Selene leaves), 38 (Selene branches) and 18 (Helios branches) children.
+TREE_HASH_SHAPES = [(1, 228, "selene_leaf"), (1, 38, "selene_branch"), (0, 18, "helios_branch")]
+
+
+def tree_layer_scalars(n, count, seed):
+    """count nodes of one tree layer with n children each: uniform 255-bit scalars (the children's
+    hashes; the library and the reference reduce them mod the group order).  uint8 (count, n, 32)."""
+    rng = np.random.default_rng(seed)
+    sc = rng.integers(0, 256, size=(count, n, 32), dtype=np.uint8)
+    sc[:, :, 31] &= 0x7F
+    return sc
+
+
 def fcmpp_shapes():
     """[(curve, n, batch)] of the FCMP++ verification MSMs: Selene 522 + 80 b and Helios 265 + 80 b
     for b in 1, 2, 4, 10 (src/benchmark_fcmpp.cpp:347-365)."""

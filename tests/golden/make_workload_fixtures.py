@@ -12,6 +12,9 @@ Inputs are regenerated from seeds by helioselene_amd/workloads.py, the generator
   own Jacobian structs, encoded by helios_tobytes.
 * ``fcmpp``: the FCMP++ verification shapes of src/benchmark_fcmpp.cpp:347-365 on the reference's
   own doubling-chain bases (Z != 1) and benchmark scalars.
+* ``tree_hash``: FCMP++ tree hashes (src/benchmark_fcmpp.cpp:133-198): nodes of a layer over the
+  doubling-chain generators (Selene 228 and 38, Helios 18) with seeded uniform scalars, each the
+  reference's MSM of one node's children.
 * ``fixed_degenerate``: helios_msm_fixed / selene_msm_fixed (helios/include/helios_msm_fixed.h:60-155)
   on repeated bases with repeated scalars, where its madd-2007-bl meets an accumulator equal to the
   table entry, next to the exact sum.
@@ -54,6 +57,22 @@ def fcmpp_case(c, n, b):
             "ref_result": O.ref_msm_jac(c, sc, pts).hex()}
 
 
+TREE_NODES = 4
+
+
+def tree_hash_cases():
+    out = []
+    for c, n, name in W.TREE_HASH_SHAPES:
+        pts = O.ref_dbl_chain(c, n)
+        seed = 7000 + n
+        sc = W.tree_layer_scalars(n, TREE_NODES, seed)
+        for h in range(TREE_NODES):
+            out.append({"kind": "tree_hash", "curve": O.CURVES[c], "n": n, "layer": name, "seed": seed, "index": h,
+                        "generator": "workloads.doubling_chain, workloads.tree_layer_scalars",
+                        "ref_result": O.ref_msm_jac(c, sc[h], pts).hex()})
+    return out
+
+
 def fixed_degenerate_cases():
     out = []
     for c in (0, 1):
@@ -89,6 +108,7 @@ def main():
     for c, n, b in W.fcmpp_shapes():
         runs.append(fcmpp_case(c, n, b))
         print(runs[-1], flush=True)
+    runs += tree_hash_cases()
     runs += fixed_degenerate_cases()
     with open(os.path.join(HERE, "workload_runs.json"), "w") as f:
         json.dump(runs, f, indent=1)

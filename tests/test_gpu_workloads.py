@@ -100,6 +100,31 @@ def test_fcmpp_shapes_single_and_batched():
     assert [o.hex() for o, _ in res] == want
 
 
+def test_fcmpp_tree_hash_layers():
+    """FCMP++ tree hashes (src/benchmark_fcmpp.cpp:133-198): nodes of a layer over its fixed
+    generators (Selene 228 and 38, Helios 18, doubling chains with Z != 1) as plain MSMs, as one
+    batch, and over a resident fixed-base set of the generators by both of its methods, against the
+    compiled reference's bytes."""
+    runs = _runs("tree_hash")
+    for c, n, name in W.TREE_HASH_SHAPES:
+        rs = sorted([r for r in runs if r["curve"] == O.CURVES[c] and r["n"] == n], key=lambda r: r["index"])
+        want = [r["ref_result"] for r in rs]
+        pts = W.doubling_chain(c, n)
+        sc = W.tree_layer_scalars(n, len(rs), rs[0]["seed"])
+        print(name, "plain", flush=True)
+        assert [hs.msm(c, sc[h], pts)[0].hex() for h in range(len(rs))] == want, name
+        print(name, "batch", flush=True)
+        assert [o.hex() for o, _ in hs.msm_batch([(c, sc[h], pts) for h in range(len(rs))])] == want, name
+        fs = hs.FixedSet(c, pts)
+        try:
+            for method in (hs.FIXED_COMB, hs.FIXED_PIPPENGER):
+                print(name, "fixed", method, flush=True)
+                out, _ = fs.msm(sc, method=method)
+                assert [o.hex() for o in out] == want, (name, method)
+        finally:
+            fs.close()
+
+
 def test_fcmpp_reference_chain_structs():
     """The reference's own doubling-chain structs (its limbs) give the same bytes as the
     workload generator's canonical structs."""

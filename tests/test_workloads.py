@@ -80,6 +80,22 @@ def test_fcmpp_fixture_matches_oracle():
         assert O.msm_naive(c, sc, pb, n).hex() == r["ref_result"]
 
 
+def test_tree_hash_fixture_matches_oracle():
+    """The committed reference bytes of the FCMP++ tree-hash nodes (Selene 228 and 38, Helios 18
+    doubling-chain generators, seeded uniform scalars) against the oracle's MSM and naive sum."""
+    runs = _runs("tree_hash")
+    assert sorted({(r["curve"], r["n"]) for r in runs}) == [("helios", 18), ("selene", 38), ("selene", 228)]
+    for c, n, name in W.TREE_HASH_SHAPES:
+        rs = sorted([r for r in runs if r["curve"] == O.CURVES[c] and r["n"] == n], key=lambda r: r["index"])
+        assert [r["index"] for r in rs] == list(range(len(rs))) and rs[0]["layer"] == name
+        pts = W.doubling_chain(c, n)
+        pb = b"".join(_affine_bytes(c, pts[i]) for i in range(n))
+        sc = W.tree_layer_scalars(n, len(rs), rs[0]["seed"])
+        for h, r in enumerate(rs):
+            assert O.msm(c, sc[h].tobytes(), pb, n).hex() == r["ref_result"], (name, h)
+            assert O.msm_naive(c, sc[h].tobytes(), pb, n).hex() == r["ref_result"], (name, h)
+
+
 @pytest.mark.parametrize("c", [0, 1])
 def test_plain_inputs_shape(c):
     n = 1 << 14
