@@ -441,12 +441,16 @@ struct HostStage
     unsigned char *pin[2] = {nullptr, nullptr}; // pinned upload slots (UPLOAD_CHUNK each)
     hipEvent_t pin_ev[2] = {nullptr, nullptr};  // the last DMA out of each slot
     bool pin_used[2] = {false, false};
+    bool pin_dirty = false; // the slots hold data of an upload not yet erased
 
-    // dev <- src (host, any memory), stream-ordered on st; returns 0 or an error code
-    int upload(void *dev, const void *src, size_t bytes)
+    // dev <- src (host, any memory), ordered on stream s (the slot's own stream by default);
+    // returns 0 or an error code
+    int upload(void *dev, const void *src, size_t bytes, hipStream_t s = nullptr)
     {
+        if (!s)
+            s = st;
         if (bytes < UPLOAD_DIRECT)
-            return hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess
+            return hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess
                        ? 0
                        : fail(HS_GPU_EDEVICE, "input upload failed");
         for (int k = 0; k < 2; k++)
@@ -467,17 +471,21 @@ struct HostStage
             if (pin_used[k] && hipEventSynchronize(pin_ev[k]) != hipSuccess) // the slot's last DMA is done
                 return fail(HS_GPU_EDEVICE, "input upload failed");
             CopyPool::get().copy(pin[k], (const char *)src + off, len);
-            if (hipMemcpyAsync((char *)dev + off, pin[k], len, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipEventRecord(pin_ev[k], st) != hipSuccess)
+            if (hipMemcpyAsync((char *)dev + off, pin[k], len, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipEventRecord(pin_ev[k], s) != hipSuccess)
                 return fail(HS_GPU_EDEVICE, "input upload failed");
             pin_used[k] = true;
+            pin_dirty = true;
         }
         return 0;
     }
 
-    // zeroes the pinned slots once their DMAs are done (they may hold scalars)
+    // zeroes the pinned slots once their DMAs are done, when an upload has passed through them
+    // since the last erase (they may hold scalars)
     void erase_pinned()
     {
+        if (!pin_dirty)
+            return;
         for (int k = 0; k < 2; k++)
             if (pin[k])
             {
@@ -485,6 +493,7 @@ struct HostStage
                     hipEventSynchronize(pin_ev[k]);
                 secure_erase(pin[k], UPLOAD_CHUNK);
             }
+        pin_dirty = false;
     }
 };
 
@@ -1637,11 +1646,19 @@ int fixed_msm_host(hs_gpu_fixed &f, const unsigned char *scalars, size_t count, 
     }
     unsigned char *o32 = dout.as<unsigned char>();
     hs_gpu_jacobian *oj = (hs_gpu_jacobian *)(o32 + count * 32);
-    hipError_t e = hipMemcpyAsync(ds.p, scalars, count * f.n * 32, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
+    // the scalars go up through a staging slot's pinned buffers (taken for this call only)
+    HostStage *hsg = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        if ((r = cx.init()) == 0 && !(hsg = take_stage(cx)))
+            r = HS_GPU_EDEVICE;
+    }
+    if (!r)
+        r = hsg->upload(ds.p, scalars, count * f.n * 32, st);
+    if (hsg && (flags & HS_GPU_ERASE))
+        hsg->erase_pinned();
+    if (!r)
         r = fixed_enqueue(f, ds.p, count, flags, method, o32, oj, st);
-    else
-        r = fail(HS_GPU_EDEVICE, std::string("scalar upload: ") + hipGetErrorString(e));
     if (!r && out32 && hipMemcpyAsync(out32, o32, count * 32, hipMemcpyDeviceToHost, st) != hipSuccess)
         r = fail(HS_GPU_EDEVICE, "result copy failed");
     if (!r && results && hipMemcpyAsync(results, oj, count * sizeof(hs_gpu_jacobian), hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -1659,6 +1676,11 @@ int fixed_msm_host(hs_gpu_fixed &f, const unsigned char *scalars, size_t count, 
     }
     if (hipStreamSynchronize(st) != hipSuccess && !r)
         r = fail(HS_GPU_EDEVICE, "fixed-base MSM failed");
+    if (hsg)
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        cx.stages_free.push_back(hsg);
+    }
     if (f.pip && method == HS_GPU_FIXED_PIPPENGER)
     {
         hs_gpu_batch_destroy(f.pip); // its descriptors point at the buffers freed below
