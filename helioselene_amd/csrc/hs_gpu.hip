@@ -2563,10 +2563,14 @@ int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
     }
     Ctx &cx = ctx();
     int r;
+    // the inputs go up through a staging slot's pinned buffers (taken for this call only)
+    HostStage *hsg = nullptr;
     {
         std::lock_guard<std::mutex> lk(cx.mu);
         if ((r = cx.init()))
             return r;
+        if (!(hsg = take_stage(cx)))
+            return HS_GPU_EDEVICE;
     }
     // stage every input in one device buffer, run the batch, copy the outputs back; every exit
     // below goes through the cleanup at the end
@@ -2583,10 +2587,8 @@ int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
         for (size_t i = 0; i < count && !r; i++)
         {
             const size_t n = descs[i].n;
-            if (n && (hipMemcpyAsync(sc_base + off * 32, descs[i].scalars, n * 32, hipMemcpyHostToDevice, st) != hipSuccess ||
-                      hipMemcpyAsync(pt_base + off * sizeof(hs_gpu_jacobian), descs[i].points, n * sizeof(hs_gpu_jacobian),
-                                     hipMemcpyHostToDevice, st) != hipSuccess))
-                r = fail(HS_GPU_EDEVICE, "batch input upload failed");
+            if (n && !(r = hsg->upload(sc_base + off * 32, descs[i].scalars, n * 32, st)))
+                r = hsg->upload(pt_base + off * sizeof(hs_gpu_jacobian), descs[i].points, n * sizeof(hs_gpu_jacobian), st);
             dd[i].scalars = sc_base + off * 32;
             dd[i].points = (const hs_gpu_jacobian *)(pt_base + off * sizeof(hs_gpu_jacobian));
             dd[i].out32 = o32_base + i * 32;
@@ -2595,6 +2597,8 @@ int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
         }
         if (!r && hipStreamSynchronize(st) != hipSuccess)
             r = fail(HS_GPU_EDEVICE, "batch input upload failed");
+        if (flags & HS_GPU_ERASE)
+            hsg->erase_pinned(); // also after a failed upload
         if (!r)
             r = batch_create(&b, dd.data(), count, flags);
         if (!r)
@@ -2635,6 +2639,10 @@ int hs_gpu_msm_batch(const hs_gpu_msm_desc *descs, size_t count, unsigned flags)
         }
     }
     hipStreamSynchronize(st);
+    {
+        std::lock_guard<std::mutex> lk(cx.mu);
+        cx.stages_free.push_back(hsg);
+    }
     hs_gpu_batch_destroy(b);
     in.release();
     outb.release();

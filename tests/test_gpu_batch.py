@@ -80,16 +80,19 @@ def test_batch_mix_vs_compiled_reference():
     assert [g[0] for g in got] == want
 
 
-def test_batch_matches_plain_path():
+@pytest.mark.parametrize("flags", [0, hs.ERASE])
+def test_batch_matches_plain_path(flags):
+    """Host-buffer batch against the plain path; the 150000-point element's upload (4.8 MB of
+    scalars, 18 MB of points) goes through the pinned staging slots, the points in two pieces."""
     items = []
-    for k, n in enumerate((100, 4096, 12000)):
+    for k, n in enumerate((100, 4096, 12000, 150000)):
         c = k % 2
         rng = np.random.default_rng(k)
         sc = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
         sc[:, 31] &= 0x7F
         pts, _ = hs.map_to_curve(c, rng.integers(0, 256, size=n * 32, dtype=np.uint8).tobytes())
         items.append((c, sc, pts))
-    got = hs.msm_batch(items)
+    got = hs.msm_batch(items, flags=flags)
     for (c, sc, pts), (out, _) in zip(items, got):
         assert out == hs.msm(c, sc, pts)[0]
 

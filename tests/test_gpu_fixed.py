@@ -142,6 +142,28 @@ def test_pedersen_batch_4096_generators(c):
         assert comb[k] == want and pip[k] == want, k
 
 
+@pytest.mark.parametrize("flags", [0, hs.ERASE])
+def test_fixed_set_large_host_upload(flags):
+    """160 rows over a set of 4097 bases: 21 MB of host scalars, uploaded in two 16 MiB pieces
+    through the pinned staging slots (with ERASE wiped afterwards).  Every row is checked against
+    the Pippenger method over the same set, three rows against the plain MSM, and the Pedersen form
+    (which always erases) against the rows."""
+    c, m, K = 1, 4097, 160
+    rng = np.random.default_rng(77 + flags)
+    pts, _ = hs.map_to_curve(c, rng.integers(0, 256, size=m * 32, dtype=np.uint8).tobytes())
+    S = rng.integers(0, 256, size=(K, m, 32), dtype=np.uint8)
+    S[:, :, 31] &= 0x7F
+    fs = hs.FixedSet(c, pts)
+    comb, _ = fs.msm(S, flags=flags, method=hs.FIXED_COMB)
+    pip, _ = fs.msm(S, flags=flags, method=hs.FIXED_PIPPENGER)
+    ped, _ = fs.pedersen(S[:, 0], S[:, 1:], method=hs.FIXED_COMB)
+    fs.close()
+    assert comb == pip and ped == comb
+    # rows 0, 127 (it straddles the two upload pieces) and 159
+    for k in (0, 127, K - 1):
+        assert comb[k] == hs.msm(c, S[k], pts)[0], k
+
+
 @pytest.mark.parametrize("c", [0, 1])
 def test_pedersen_batch_vs_compiled_reference(c):
     """The Pedersen-batch shape (H + 2^12 SSWU generators held resident, uniform values and
