@@ -491,7 +491,9 @@ def run_pedersen(hs, torch, args, count=1024, m=4096):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             fs = hs.FixedSet(c, d_points=pts.data_ptr(), n=m + 1, stream=sh)
-            res[f"{['helios', 'selene'][c]}_table_build_ms"] = (time.perf_counter() - t0) * 1e3
+            torch.cuda.synchronize()
+            build = time.perf_counter() - t0
+            res[f"{['helios', 'selene'][c]}_table_build_ms"] = build * 1e3
             s = rng.integers(0, 256, size=(count, m + 1, 32), dtype=np.uint8)
             s[:, :, 31] &= 0x7F
             ds = torch.from_numpy(s).to(dev)
@@ -508,6 +510,8 @@ def run_pedersen(hs, torch, args, count=1024, m=4096):
                 tm = float(np.median(t))
                 res[f"{['helios', 'selene'][c]}_{name}_ms"] = tm * 1e3
                 res[f"{['helios', 'selene'][c]}_{name}_points_per_s"] = count * (m + 1) / tm
+                if name == "comb":
+                    res[f"{['helios', 'selene'][c]}_comb_points_per_s_incl_build"] = count * (m + 1) / (tm + build)
                 outs[(c, name)] = out.cpu().numpy()
             fs.close()
             # the width-5 affine tables of the workload as written ([1P..16P] per generator, 52

@@ -153,31 +153,38 @@ __global__ void __launch_bounds__(256) k_comb_teeth(const uint32_t *__restrict__
     store_aff<F>(teeth, t, a);
 }
 
-// Comb tables, step 2: entry e of base i is Q_15 + sum_{j<15} (bit j of e ? Q_j : -Q_j), one
-// thread per entry.  No entry of a non-identity base is the identity: |2^240 + sum_{j<15} +-2^(16 j)|
-// lies in (0, 2^241), below the group order.
+// Comb tables, step 2: entry e of base i, Q_15 + sum_{j<15} (bit j of e ? Q_j : -Q_j), is
+// A_(e mod 256) + B_(e >> 8) with
+//   A_l = sum_{j<8} (bit j of l ? Q_j : -Q_j)                     (256 per base)
+//   B_h = Q_15 + sum_{j<7} (bit j of h ? Q_(8+j) : -Q_(8+j))       (128 per base)
+// Step 2a: the half tables as canonical affine entries, A then B (COMB_HALF per base), one thread
+// per half entry.
+constexpr uint32_t COMB_HALF = 384;
 template <class C>
-__global__ void __launch_bounds__(256) k_comb_tables(const uint32_t *__restrict__ teeth, uint32_t n,
-                                                     uint32_t *__restrict__ tables)
+__global__ void __launch_bounds__(256) k_comb_halves(const uint32_t *__restrict__ teeth, uint32_t n,
+                                                     uint32_t *__restrict__ halves)
 {
     using F = typename C::F;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (n << 15))
+    if (t >= n * COMB_HALF)
         return;
-    const uint32_t i = t >> 15, e = t & 0x7fffu;
+    const uint32_t i = t / COMB_HALF, u = t % COMB_HALF;
     aff q, a;
     load_aff(q, teeth, 16 * i + 15);
     bool pt = !(fe_is_zero<F>(q.x) && fe_is_zero<F>(q.y));
     if (pt)
     {
+        const bool hi = u >= 256;
+        const uint32_t bits = hi ? u - 256 : u;
         xyzz acc;
         bool acc_id = true;
         xyzz_identity<F>(acc);
-        xyzz_madd<F>(acc, acc_id, q);
-        for (int j = 0; j < 15; j++)
+        if (hi)
+            xyzz_madd<F>(acc, acc_id, q);
+        for (int j = 0; j < (hi ? 7 : 8); j++)
         {
-            load_aff(q, teeth, 16 * i + j);
-            if (!((e >> j) & 1u))
+            load_aff(q, teeth, 16 * i + (hi ? 8 : 0) + j);
+            if (!((bits >> j) & 1u))
                 fe_neg<F>(q.y, q.y);
             xyzz_madd<F>(acc, acc_id, q);
         }
@@ -188,7 +195,114 @@ __global__ void __launch_bounds__(256) k_comb_tables(const uint32_t *__restrict_
         fe_zero(a.x);
         fe_zero(a.y);
     }
-    store_aff<F>(tables, t, a);
+    store_aff<F>(halves, t, a);
+}
+
+// Step 2b: thread (i, h, r) forms the COMB_RUN entries (h << 8) | (r + 32 k) of base i as affine
+// sums A + B with one inversion for all their denominators x_B - x_A (Montgomery's trick):
+//   lambda = (y_B - y_A) / (x_B - x_A),  x = lambda^2 - x_A - x_B,  y = lambda (x_A - x) - y_A.
+// For a point of prime order no denominator is zero: A_l = +-B_h would need sum_{j<8} +-2^(16 j)
+// = +-(2^240 + sum_{8<=j<15} +-2^(16 j)) modulo the order, with both sides in (-2^241, 2^241).  A
+// thread that meets a zero denominator or an identity half entry anyway takes the general
+// addition for its entries.  Adjacent threads write adjacent entries.
+constexpr int COMB_RUN = 8;
+template <class C>
+__global__ void __launch_bounds__(256) k_comb_tables_runs(const uint32_t *__restrict__ halves, uint32_t n,
+                                                          uint32_t *__restrict__ tables)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * (128u * 32u))
+        return;
+    const uint32_t i = t >> 12, h = (t >> 5) & 127u, r = t & 31u;
+    const uint32_t a0 = COMB_HALF * i + r, out0 = (i << 15) | (h << 8) | r;
+    aff b;
+    load_aff(b, halves, COMB_HALF * i + 256 + h);
+    uint32_t bz = 0;
+#pragma unroll
+    for (int j = 0; j < NL; j++)
+        bz |= b.x.v[j] | b.y.v[j];
+    if (bz == 0)
+    {
+        // identity base: every entry is the identity
+        aff z;
+        fe_zero(z.x);
+        fe_zero(z.y);
+#pragma unroll
+        for (int k = 0; k < COMB_RUN; k++)
+            store_aff<F>(tables, out0 + 32 * k, z);
+        return;
+    }
+    fe c[COMB_RUN]; // prefix products of the denominators
+    uint32_t az = M29;
+#pragma unroll
+    for (int k = 0; k < COMB_RUN; k++)
+    {
+        aff a;
+        load_aff(a, halves, a0 + 32 * k);
+        uint32_t o = 0;
+#pragma unroll
+        for (int j = 0; j < NL; j++)
+            o |= a.x.v[j] | a.y.v[j];
+        az &= o ? M29 : 0u;
+        fe d;
+        fe_sub<F>(d, b.x, a.x);
+        if (k == 0)
+            c[0] = d;
+        else
+            fe_mul<F>(c[k], c[k - 1], d);
+    }
+    if (az == 0 || fe_is_zero<F>(c[COMB_RUN - 1]))
+    {
+        for (int k = 0; k < COMB_RUN; k++)
+        {
+            aff a, o;
+            load_aff(a, halves, a0 + 32 * k);
+            uint32_t nz = 0;
+#pragma unroll
+            for (int j = 0; j < NL; j++)
+                nz |= a.x.v[j] | a.y.v[j];
+            xyzz acc;
+            bool acc_id = true;
+            xyzz_identity<F>(acc);
+            if (nz)
+                xyzz_madd<F>(acc, acc_id, a);
+            xyzz_madd<F>(acc, acc_id, b);
+            if (acc_id || !xyzz_to_aff<F>(o, acc))
+            {
+                fe_zero(o.x);
+                fe_zero(o.y);
+            }
+            store_aff<F>(tables, out0 + 32 * k, o);
+        }
+        return;
+    }
+    fe inv;
+    fe_inv<F>(inv, c[COMB_RUN - 1]);
+#pragma unroll
+    for (int k = COMB_RUN - 1; k >= 0; k--)
+    {
+        aff a;
+        load_aff(a, halves, a0 + 32 * k);
+        fe ik, t0, lam;
+        if (k > 0)
+        {
+            fe_mul<F>(ik, inv, c[k - 1]);
+            fe_sub<F>(t0, b.x, a.x);
+            fe_mul<F>(inv, inv, t0);
+        }
+        else
+            ik = inv;
+        fe_sub<F>(t0, b.y, a.y);
+        fe_mul<F>(lam, t0, ik);
+        aff o;
+        fe_sq<F>(t0, lam);
+        fe_sub2<F>(o.x, t0, a.x, b.x);
+        fe_sub<F>(t0, a.x, o.x);
+        fe_mul<F>(o.y, lam, t0);
+        fe_sub<F>(o.y, o.y, a.y);
+        store_aff<F>(tables, out0 + 32 * k, o);
+    }
 }
 
 // Tables given in the reference's helios_affine layout (possibly lazy limbs): canonicalise.

@@ -1472,15 +1472,19 @@ template <class C> int fixed_build_comb(hs_gpu_fixed &f, hipStream_t st)
 {
     constexpr int E = hs::FixedW<hs::FIXED_COMB16>::entries;
     int r;
-    if ((r = f.tables.reserve_exact((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * 64)) ||
+    // the per-call scalar workspace stages the affine bases, then (once the teeth are built from
+    // them) the half tables
+    if ((r = f.tables.reserve_exact((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * hs::COMB_HALF * 64)) ||
         (r = f.sums.reserve((size_t)f.n * 16 * 64)))
         return r;
     hipLaunchKernelGGL(hs::k_fixed_bases<C>, dim3((f.n + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
                        f.scal.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_comb_teeth<C>, dim3((16 * f.n + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
                        f.n, f.sums.as<uint32_t>());
-    const uint32_t t = f.n * (uint32_t)E;
-    hipLaunchKernelGGL(hs::k_comb_tables<C>, dim3((t + 255) / 256), dim3(256), 0, st, f.sums.as<uint32_t>(), f.n,
+    const uint32_t th = f.n * hs::COMB_HALF, t = f.n * (128u * 32u);
+    hipLaunchKernelGGL(hs::k_comb_halves<C>, dim3((th + 255) / 256), dim3(256), 0, st, f.sums.as<uint32_t>(), f.n,
+                       f.scal.as<uint32_t>());
+    hipLaunchKernelGGL(hs::k_comb_tables_runs<C>, dim3((t + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(), f.n,
                        f.tables.as<uint32_t>());
     HS_CHECK(hipGetLastError());
     f.wb = hs::FIXED_COMB16;
