@@ -198,74 +198,73 @@ __global__ void __launch_bounds__(256) k_comb_halves(const uint32_t *__restrict_
     store_aff<F>(halves, t, a);
 }
 
-// Step 2b: thread (i, h, r) forms the COMB_RUN entries (h << 8) | (r + 32 k) of base i as affine
-// sums A + B with one inversion for all their denominators x_B - x_A (Montgomery's trick):
-//   lambda = (y_B - y_A) / (x_B - x_A),  x = lambda^2 - x_A - x_B,  y = lambda (x_A - x) - y_A.
-// For a point of prime order no denominator is zero: A_l = +-B_h would need sum_{j<8} +-2^(16 j)
-// = +-(2^240 + sum_{8<=j<15} +-2^(16 j)) modulo the order, with both sides in (-2^241, 2^241).  A
-// thread that meets a zero denominator or an identity half entry anyway takes the general
-// addition for its entries.  Adjacent threads write adjacent entries.
+// COMB_RUN table entries B + A_k, A_k = A[a0 + a_step k] (canonical affine, all-zero = identity),
+// written to out[o0 + o_step k] for the k whose bit is set in `valid`.  One inversion serves all
+// the denominators x_B - x_A of the terms that are not the identity (Montgomery's trick):
+//   lambda = (y_B - y_A) / (x_B - x_A),  x = lambda^2 - x_A - x_B,  y = lambda (x_A - x) - y_A;
+// an identity term gives the other one.  Should a denominator be zero (B = +-A_k), every entry takes
+// the general addition.
 constexpr int COMB_RUN = 8;
-template <class C>
-__global__ void __launch_bounds__(256) k_comb_tables_runs(const uint32_t *__restrict__ halves, uint32_t n,
-                                                          uint32_t *__restrict__ tables)
+__device__ __forceinline__ bool aff_is_zero_words(const aff &a)
 {
-    using F = typename C::F;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * (128u * 32u))
-        return;
-    const uint32_t i = t >> 12, h = (t >> 5) & 127u, r = t & 31u;
-    const uint32_t a0 = COMB_HALF * i + r, out0 = (i << 15) | (h << 8) | r;
-    aff b;
-    load_aff(b, halves, COMB_HALF * i + 256 + h);
-    uint32_t bz = 0;
+    uint32_t o = 0;
 #pragma unroll
     for (int j = 0; j < NL; j++)
-        bz |= b.x.v[j] | b.y.v[j];
-    if (bz == 0)
+        o |= a.x.v[j] | a.y.v[j];
+    return o == 0;
+}
+
+template <class F>
+__device__ __forceinline__ void pair_sums(const aff &b, const uint32_t *__restrict__ A, uint32_t a0, uint32_t a_step,
+                                          uint32_t *__restrict__ out, uint32_t o0, uint32_t o_step, uint32_t valid)
+{
+    if (aff_is_zero_words(b))
     {
-        // identity base: every entry is the identity
-        aff z;
-        fe_zero(z.x);
-        fe_zero(z.y);
+        // B is the identity: the entries are the A_k
 #pragma unroll
         for (int k = 0; k < COMB_RUN; k++)
-            store_aff<F>(tables, out0 + 32 * k, z);
+            if ((valid >> k) & 1u)
+            {
+                aff a;
+                load_aff(a, A, a0 + a_step * k);
+                store_aff<F>(out, o0 + o_step * k, a);
+            }
         return;
     }
-    fe c[COMB_RUN]; // prefix products of the denominators
-    uint32_t az = M29;
+    fe one;
+    fe_zero(one);
+    one.v[0] = 1;
+    fe c[COMB_RUN]; // prefix products of the denominators (1 for an identity A_k)
+    uint32_t az = 0; // identity A_k
 #pragma unroll
     for (int k = 0; k < COMB_RUN; k++)
     {
         aff a;
-        load_aff(a, halves, a0 + 32 * k);
-        uint32_t o = 0;
-#pragma unroll
-        for (int j = 0; j < NL; j++)
-            o |= a.x.v[j] | a.y.v[j];
-        az &= o ? M29 : 0u;
+        load_aff(a, A, a0 + a_step * k);
         fe d;
         fe_sub<F>(d, b.x, a.x);
+        if (aff_is_zero_words(a))
+        {
+            az |= 1u << k;
+            d = one;
+        }
         if (k == 0)
             c[0] = d;
         else
             fe_mul<F>(c[k], c[k - 1], d);
     }
-    if (az == 0 || fe_is_zero<F>(c[COMB_RUN - 1]))
+    if (fe_is_zero<F>(c[COMB_RUN - 1]))
     {
         for (int k = 0; k < COMB_RUN; k++)
         {
+            if (!((valid >> k) & 1u))
+                continue;
             aff a, o;
-            load_aff(a, halves, a0 + 32 * k);
-            uint32_t nz = 0;
-#pragma unroll
-            for (int j = 0; j < NL; j++)
-                nz |= a.x.v[j] | a.y.v[j];
+            load_aff(a, A, a0 + a_step * k);
             xyzz acc;
             bool acc_id = true;
             xyzz_identity<F>(acc);
-            if (nz)
+            if (!aff_is_zero_words(a))
                 xyzz_madd<F>(acc, acc_id, a);
             xyzz_madd<F>(acc, acc_id, b);
             if (acc_id || !xyzz_to_aff<F>(o, acc))
@@ -273,7 +272,7 @@ __global__ void __launch_bounds__(256) k_comb_tables_runs(const uint32_t *__rest
                 fe_zero(o.x);
                 fe_zero(o.y);
             }
-            store_aff<F>(tables, out0 + 32 * k, o);
+            store_aff<F>(out, o0 + o_step * k, o);
         }
         return;
     }
@@ -283,26 +282,114 @@ __global__ void __launch_bounds__(256) k_comb_tables_runs(const uint32_t *__rest
     for (int k = COMB_RUN - 1; k >= 0; k--)
     {
         aff a;
-        load_aff(a, halves, a0 + 32 * k);
+        load_aff(a, A, a0 + a_step * k);
+        const bool zk = (az >> k) & 1u;
         fe ik, t0, lam;
         if (k > 0)
         {
             fe_mul<F>(ik, inv, c[k - 1]);
             fe_sub<F>(t0, b.x, a.x);
+            if (zk)
+                t0 = one;
             fe_mul<F>(inv, inv, t0);
         }
         else
             ik = inv;
-        fe_sub<F>(t0, b.y, a.y);
-        fe_mul<F>(lam, t0, ik);
-        aff o;
-        fe_sq<F>(t0, lam);
-        fe_sub2<F>(o.x, t0, a.x, b.x);
-        fe_sub<F>(t0, a.x, o.x);
-        fe_mul<F>(o.y, lam, t0);
-        fe_sub<F>(o.y, o.y, a.y);
-        store_aff<F>(tables, out0 + 32 * k, o);
+        if ((valid >> k) & 1u)
+        {
+            aff o;
+            fe_sub<F>(t0, b.y, a.y);
+            fe_mul<F>(lam, t0, ik);
+            fe_sq<F>(t0, lam);
+            fe_sub2<F>(o.x, t0, a.x, b.x);
+            fe_sub<F>(t0, a.x, o.x);
+            fe_mul<F>(o.y, lam, t0);
+            fe_sub<F>(o.y, o.y, a.y);
+            store_aff<F>(out, o0 + o_step * k, zk ? b : o);
+        }
     }
+}
+
+// Step 2b: thread (i, h, r) forms the COMB_RUN entries (h << 8) | (r + 32 k) of base i as B_h +
+// A_(r + 32 k); adjacent threads write adjacent entries.  For a point of prime order no half entry
+// is the identity and no denominator is zero: A_l = +-B_h would need sum_{j<8} +-2^(16 j) =
+// +-(2^240 + sum_{8<=j<15} +-2^(16 j)) modulo the order, with both sides in (-2^241, 2^241).  An
+// identity base has all-zero half tables and gets all-zero entries.
+template <class C>
+__global__ void __launch_bounds__(256) k_comb_tables_runs(const uint32_t *__restrict__ halves, uint32_t n,
+                                                          uint32_t *__restrict__ tables)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * (128u * 32u))
+        return;
+    const uint32_t i = t >> 12, h = (t >> 5) & 127u, r = t & 31u;
+    aff b;
+    load_aff(b, halves, COMB_HALF * i + 256 + h);
+    pair_sums<F>(b, halves, COMB_HALF * i + r, 32, tables, (i << 15) | (h << 8) | r, 32, 0xffu);
+}
+
+// Windowed tables (entries (k+1) P, k < E = 2^(WB-1), WB >= 8) by the same construction: with
+// m = k + 1 = 32 a + b (0 <= b < 32, 0 <= a <= E / 32), entry m is H_a + L_b over the half tables
+// L_b = b P and H_a = 32 a P (the identity at b = 0 and a = 0), WIN_HALF(E) entries per base,
+// L first.  For a point of prime order the denominators of the sums with a, b > 0 are nonzero
+// (32 a != +-b modulo the order); a sum with a = 0 or b = 0 is its other term.
+__host__ __device__ constexpr uint32_t win_half(uint32_t E) { return 32 + E / 32 + 1; }
+
+// Step 1: one thread per (base, half entry)
+template <class C, int E>
+__global__ void __launch_bounds__(256) k_win_halves(const uint32_t *__restrict__ base_aff, uint32_t n,
+                                                    uint32_t *__restrict__ halves)
+{
+    using F = typename C::F;
+    constexpr uint32_t NH = win_half(E);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * NH)
+        return;
+    const uint32_t i = t / NH, u = t % NH, k = u < 32 ? u : 32 * (u - 32);
+    aff base, a;
+    load_aff(base, base_aff, i);
+    bool pt = k != 0 && !aff_is_zero_words(base);
+    if (pt)
+    {
+        xyzz b, m;
+        xyzz_from_aff<F>(b, base);
+        xyzz_mul_small<F>(m, b, k);
+        pt = xyzz_to_aff<F>(a, m);
+    }
+    if (!pt)
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+    }
+    store_aff<F>(halves, t, a);
+}
+
+// Step 2: thread (i, a, r) forms the entries m = 32 a + r + 4 j (j < COMB_RUN) in [1, E] of base i;
+// adjacent threads write adjacent entries.
+template <class C, int E>
+__global__ void __launch_bounds__(256) k_win_tables_runs(const uint32_t *__restrict__ halves, uint32_t n,
+                                                         uint32_t *__restrict__ tables)
+{
+    using F = typename C::F;
+    constexpr uint32_t NH = win_half(E), NA = E / 32 + 1;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * NA * 4)
+        return;
+    const uint32_t i = t / (NA * 4), a = (t / 4) % NA, r = t & 3u;
+    uint32_t valid = 0;
+#pragma unroll
+    for (int j = 0; j < COMB_RUN; j++)
+    {
+        const uint32_t m = 32 * a + r + 4 * j;
+        valid |= (m >= 1 && m <= (uint32_t)E) ? 1u << j : 0u;
+    }
+    if (!valid)
+        return;
+    aff h;
+    load_aff(h, halves, NH * i + 32 + a);
+    // entry m lives at index m - 1; for m = 0 (a = r = 0, j = 0: not valid) the index is unused
+    pair_sums<F>(h, halves, NH * i + r, 4, tables, E * i + 32 * a + r - 1, 4, valid);
 }
 
 // Tables given in the reference's helios_affine layout (possibly lazy limbs): canonicalise.

@@ -1501,9 +1501,27 @@ template <class C, int WB> int fixed_build_tables_w(hs_gpu_fixed &f, uint64_t *d
     hipLaunchKernelGGL(hs::k_fixed_bases<C>, dim3((f.n + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
                        f.scal.as<uint32_t>());
     const uint32_t t = f.n * (uint32_t)E;
-    hipLaunchKernelGGL((hs::k_fixed_tables<C, E>), dim3((t + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
-                       f.n, f.tables.as<uint32_t>(), d_ref_out);
-    HS_CHECK(hipGetLastError());
+    DevBuf halves; // wide tables: entries as sums over two half tables (freed after the build)
+    if (WB >= 8 && !d_ref_out && halves.reserve_exact((size_t)f.n * hs::win_half(E) * 64) == 0)
+    {
+        const uint32_t th = f.n * hs::win_half(E), tr = f.n * (E / 32 + 1) * 4;
+        hipLaunchKernelGGL((hs::k_win_halves<C, E>), dim3((th + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
+                           f.n, halves.as<uint32_t>());
+        hipLaunchKernelGGL((hs::k_win_tables_runs<C, E>), dim3((tr + 255) / 256), dim3(256), 0, st,
+                           halves.as<uint32_t>(), f.n, f.tables.as<uint32_t>());
+        const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(st);
+        halves.release();
+        if (e1 != hipSuccess || e2 != hipSuccess)
+            return fail(HS_GPU_EDEVICE, "fixed-base table build failed");
+    }
+    else
+    {
+        // one multiplication and one inversion per entry (the width-5 tables, or no room for the
+        // half tables)
+        hipLaunchKernelGGL((hs::k_fixed_tables<C, E>), dim3((t + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
+                           f.n, f.tables.as<uint32_t>(), d_ref_out);
+        HS_CHECK(hipGetLastError());
+    }
     f.wb = WB;
     return 0;
 }

@@ -234,6 +234,29 @@ def test_fixed_set_every_table_width(c, width, monkeypatch):
 
 
 @pytest.mark.parametrize("c", [0, 1])
+@pytest.mark.parametrize("width", [14, 13, 12, 10, 8])
+def test_fixed_set_windowed_tables_every_entry(c, width, monkeypatch):
+    """Every entry of the windowed tables, 1..2^(w-1) times the base: row m puts scalar m on a
+    random base (entry m, or entry 2^w - m negated for m >= 2^(w-1), so m = 1..2^(w-1) reads all of
+    them) and a random scalar on an identity base; each row against the oracle's m P."""
+    monkeypatch.setenv("HS_GPU_FIXED_WIDTH", str(width))
+    E = 1 << (width - 1)
+    P = O.gen_sswu_bases(c, 900 + width, 1)
+    pts = GH.to_jac_array(c, P + bytes(32), 2, rnd=random.Random(width))
+    rnd = np.random.default_rng(width)
+    S = np.zeros((E, 2, 32), np.uint8)
+    for m in range(1, E + 1):
+        S[m - 1, 0] = np.frombuffer(m.to_bytes(32, "little"), np.uint8)
+    S[:, 1] = rnd.integers(0, 256, size=(E, 32), dtype=np.uint8)
+    S[:, 1, 31] &= 0x7F
+    fs = hs.FixedSet(c, pts)
+    got, _ = fs.msm(S, flags=hs.EXACT, method=hs.FIXED_COMB)
+    fs.close()
+    for m in range(1, E + 1):
+        assert got[m - 1] == O.scalar_mul(c, m.to_bytes(32, "little"), P), m
+
+
+@pytest.mark.parametrize("c", [0, 1])
 def test_msm_fixed_degenerate_case_is_exact(c):
     """Repeated (base, scalar) pairs, where the reference's msm_fixed meets an accumulator equal to
     the table entry and drops the pair: the drop-in returns the exact sum (the documented
