@@ -205,6 +205,7 @@ __global__ void __launch_bounds__(256) k_comb_halves(const uint32_t *__restrict_
 // an identity term gives the other one.  Should a denominator be zero (B = +-A_k), every entry takes
 // the general addition.
 constexpr int COMB_RUN = 16;
+static_assert(COMB_RUN <= 32 && 32 % COMB_RUN == 0, "runs split the 32 low half entries evenly");
 __device__ __forceinline__ bool aff_is_zero_words(const aff &a)
 {
     uint32_t o = 0;
@@ -310,8 +311,8 @@ __device__ __forceinline__ void pair_sums(const aff &b, const uint32_t *__restri
     }
 }
 
-// Step 2b: thread (i, h, r) forms the COMB_RUN entries (h << 8) | (r + 16 k) of base i as B_h +
-// A_(r + 16 k); adjacent threads write adjacent entries.  For a point of prime order no half entry
+// Step 2b: thread (i, h, r) forms the COMB_RUN entries (h << 8) | (r + S k) of base i as B_h +
+// A_(r + S k), S = 256 / COMB_RUN; adjacent threads write adjacent entries.  For a point of prime order no half entry
 // is the identity and no denominator is zero: A_l = +-B_h would need sum_{j<8} +-2^(16 j) =
 // +-(2^240 + sum_{8<=j<15} +-2^(16 j)) modulo the order, with both sides in (-2^241, 2^241).  An
 // identity base has all-zero half tables and gets all-zero entries.
@@ -321,13 +322,14 @@ __global__ void __launch_bounds__(256) k_comb_tables_runs(const uint32_t *__rest
 {
     using F = typename C::F;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    static_assert(COMB_RUN == 16, "thread layout");
-    if (t >= n * (128u * 16u))
+    constexpr uint32_t S = 256 / COMB_RUN;
+    if (t >= n * (128u * S))
         return;
-    const uint32_t i = t >> 11, h = (t >> 4) & 127u, r = t & 15u;
+    const uint32_t i = t / (128u * S), h = (t / S) % 128u, r = t % S;
     aff b;
     load_aff(b, halves, COMB_HALF * i + 256 + h);
-    pair_sums<F>(b, halves, COMB_HALF * i + r, 16, tables, (i << 15) | (h << 8) | r, 16, 0xffffu);
+    pair_sums<F>(b, halves, COMB_HALF * i + r, S, tables, (i << 15) | (h << 8) | r, S,
+                 COMB_RUN == 32 ? 0xffffffffu : (1u << COMB_RUN) - 1u);
 }
 
 // Windowed tables (entries (k+1) P, k < E = 2^(WB-1), WB >= 8) by the same construction: with
@@ -366,8 +368,8 @@ __global__ void __launch_bounds__(256) k_win_halves(const uint32_t *__restrict__
     store_aff<F>(halves, t, a);
 }
 
-// Step 2: thread (i, a, r) forms the entries m = 32 a + r + 2 j (j < COMB_RUN) in [1, E] of base i;
-// adjacent threads write adjacent entries.
+// Step 2: thread (i, a, r) forms the entries m = 32 a + r + S j (j < COMB_RUN, S = 32 / COMB_RUN) in
+// [1, E] of base i; adjacent threads write adjacent entries.
 template <class C, int E>
 __global__ void __launch_bounds__(256) k_win_tables_runs(const uint32_t *__restrict__ halves, uint32_t n,
                                                          uint32_t *__restrict__ tables)
@@ -375,15 +377,15 @@ __global__ void __launch_bounds__(256) k_win_tables_runs(const uint32_t *__restr
     using F = typename C::F;
     constexpr uint32_t NH = win_half(E), NA = E / 32 + 1;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    static_assert(COMB_RUN == 16, "thread layout");
-    if (t >= n * NA * 2)
+    constexpr uint32_t S = 32 / COMB_RUN;
+    if (t >= n * NA * S)
         return;
-    const uint32_t i = t / (NA * 2), a = (t / 2) % NA, r = t & 1u;
+    const uint32_t i = t / (NA * S), a = (t / S) % NA, r = t % S;
     uint32_t valid = 0;
 #pragma unroll
     for (int j = 0; j < COMB_RUN; j++)
     {
-        const uint32_t m = 32 * a + r + 2 * j;
+        const uint32_t m = 32 * a + r + S * j;
         valid |= (m >= 1 && m <= (uint32_t)E) ? 1u << j : 0u;
     }
     if (!valid)
@@ -391,7 +393,7 @@ __global__ void __launch_bounds__(256) k_win_tables_runs(const uint32_t *__restr
     aff h;
     load_aff(h, halves, NH * i + 32 + a);
     // entry m lives at index m - 1; for m = 0 (a = r = 0, j = 0: not valid) the index is unused
-    pair_sums<F>(h, halves, NH * i + r, 2, tables, E * i + 32 * a + r - 1, 2, valid);
+    pair_sums<F>(h, halves, NH * i + r, S, tables, E * i + 32 * a + r - 1, S, valid);
 }
 
 // Tables given in the reference's helios_affine layout (possibly lazy limbs): canonicalise.

@@ -1481,7 +1481,7 @@ template <class C> int fixed_build_comb(hs_gpu_fixed &f, hipStream_t st)
                        f.scal.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_comb_teeth<C>, dim3((16 * f.n + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
                        f.n, f.sums.as<uint32_t>());
-    const uint32_t th = f.n * hs::COMB_HALF, t = f.n * (128u * 16u);
+    const uint32_t th = f.n * hs::COMB_HALF, t = f.n * (128u * (256u / hs::COMB_RUN));
     hipLaunchKernelGGL(hs::k_comb_halves<C>, dim3((th + 255) / 256), dim3(256), 0, st, f.sums.as<uint32_t>(), f.n,
                        f.scal.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_comb_tables_runs<C>, dim3((t + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(), f.n,
@@ -1504,7 +1504,7 @@ template <class C, int WB> int fixed_build_tables_w(hs_gpu_fixed &f, uint64_t *d
     DevBuf halves; // wide tables: entries as sums over two half tables (freed after the build)
     if (WB >= 8 && !d_ref_out && halves.reserve_exact((size_t)f.n * hs::win_half(E) * 64) == 0)
     {
-        const uint32_t th = f.n * hs::win_half(E), tr = f.n * (E / 32 + 1) * 2;
+        const uint32_t th = f.n * hs::win_half(E), tr = f.n * (E / 32 + 1) * (32u / hs::COMB_RUN);
         hipLaunchKernelGGL((hs::k_win_halves<C, E>), dim3((th + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
                            f.n, halves.as<uint32_t>());
         hipLaunchKernelGGL((hs::k_win_tables_runs<C, E>), dim3((tr + 255) / 256), dim3(256), 0, st,
