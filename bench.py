@@ -19,7 +19,7 @@ The same line carries:
   * `batched_mix`: 256 Helios + 256 Selene MSMs of log-uniform sizes in [64, 2^14], plain
     per-MSM launches against one batch plan replayed from a hipGraph;
   * `pedersen_batch`: 1024 commitments per curve over H + 2^12 resident generators, both
-    fixed-base methods;
+    fixed-base methods and the width-5 tables, with each table build timed to completion;
   * `serving`: Poisson request stream through the native server against plain per-request
     launches, p50/p99 latency and saturated throughput;
   * `host_slot_points_per_s`: the drop-in slot with host buffers (PCIe upload included);
