@@ -1502,7 +1502,20 @@ template <class C, int WB> int fixed_build_tables_w(hs_gpu_fixed &f, uint64_t *d
                        f.scal.as<uint32_t>());
     const uint32_t t = f.n * (uint32_t)E;
     DevBuf halves; // wide tables: entries as sums over two half tables (freed after the build)
-    if (WB >= 8 && !d_ref_out && halves.reserve_exact((size_t)f.n * hs::win_half(E) * 64) == 0)
+    bool use_halves = false;
+    if (WB >= 8 && !d_ref_out)
+    {
+        const std::string err = g_err;
+        use_halves = halves.reserve_exact((size_t)f.n * hs::win_half(E) * 64) == 0;
+        if (!use_halves)
+        {
+            // no room for the half tables: the entries are built one by one instead, which is not
+            // an error (clear what the refused allocation recorded)
+            (void)hipGetLastError();
+            g_err = err;
+        }
+    }
+    if (use_halves)
     {
         const uint32_t th = f.n * hs::win_half(E), tr = f.n * (E / 32 + 1) * (32u / hs::COMB_RUN);
         hipLaunchKernelGGL((hs::k_win_halves<C, E>), dim3((th + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),

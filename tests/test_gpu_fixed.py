@@ -256,6 +256,28 @@ def test_fixed_set_windowed_tables_every_entry(c, width, monkeypatch):
         assert got[m - 1] == O.scalar_mul(c, m.to_bytes(32, "little"), P), m
 
 
+def test_windowed_tables_without_room_for_half_tables(monkeypatch):
+    """Under a device cap that leaves room for a set's 8-bit-window tables but not for the
+    temporary half tables, the tables are built entry by entry instead: same results."""
+    monkeypatch.setenv("HS_GPU_FIXED_WIDTH", "8")
+    c, n, K = 0, 3000, 2
+    pb = O.gen_sswu_bases(c, 950, n)
+    pts = GH.to_jac_array(c, pb, n)
+    rows = [O.gen_scalars(c, 960 + k, n, edge_frac=0.05) for k in range(K)]
+    S = np.frombuffer(b"".join(rows), np.uint8).reshape(K, n, 32)
+    hs.trim()
+    tables = n * 128 * 64
+    hs.set_device_limit(hs.device_bytes() + tables + (2 << 20))  # the half tables need 7.1 MB
+    try:
+        fs = hs.FixedSet(c, pts)
+    finally:
+        hs.set_device_limit(0)
+    got, _ = fs.msm(S, flags=hs.EXACT, method=hs.FIXED_COMB)
+    fs.close()
+    for k in range(K):
+        assert got[k] == O.msm_naive(c, rows[k], pb, n), k
+
+
 @pytest.mark.parametrize("c", [0, 1])
 def test_msm_fixed_degenerate_case_is_exact(c):
     """Repeated (base, scalar) pairs, where the reference's msm_fixed meets an accumulator equal to
