@@ -157,46 +157,8 @@ __global__ void __launch_bounds__(256) k_comb_teeth(const uint32_t *__restrict__
 // A_(e mod 256) + B_(e >> 8) with
 //   A_l = sum_{j<8} (bit j of l ? Q_j : -Q_j)                     (256 per base)
 //   B_h = Q_15 + sum_{j<7} (bit j of h ? Q_(8+j) : -Q_(8+j))       (128 per base)
-// Step 2a: the half tables as canonical affine entries, A then B (COMB_HALF per base), one thread
-// per half entry.
+// The half tables come from quarter tables the same way (step 2a below).
 constexpr uint32_t COMB_HALF = 384;
-template <class C>
-__global__ void __launch_bounds__(256) k_comb_halves(const uint32_t *__restrict__ teeth, uint32_t n,
-                                                     uint32_t *__restrict__ halves)
-{
-    using F = typename C::F;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * COMB_HALF)
-        return;
-    const uint32_t i = t / COMB_HALF, u = t % COMB_HALF;
-    aff q, a;
-    load_aff(q, teeth, 16 * i + 15);
-    bool pt = !(fe_is_zero<F>(q.x) && fe_is_zero<F>(q.y));
-    if (pt)
-    {
-        const bool hi = u >= 256;
-        const uint32_t bits = hi ? u - 256 : u;
-        xyzz acc;
-        bool acc_id = true;
-        xyzz_identity<F>(acc);
-        if (hi)
-            xyzz_madd<F>(acc, acc_id, q);
-        for (int j = 0; j < (hi ? 7 : 8); j++)
-        {
-            load_aff(q, teeth, 16 * i + (hi ? 8 : 0) + j);
-            if (!((bits >> j) & 1u))
-                fe_neg<F>(q.y, q.y);
-            xyzz_madd<F>(acc, acc_id, q);
-        }
-        pt = !acc_id && xyzz_to_aff<F>(a, acc);
-    }
-    if (!pt)
-    {
-        fe_zero(a.x);
-        fe_zero(a.y);
-    }
-    store_aff<F>(halves, t, a);
-}
 
 // COMB_RUN table entries B + A_k, A_k = A[a0 + a_step k] (canonical affine, all-zero = identity),
 // written to out[o0 + o_step k] for the k whose bit is set in `valid`.  One inversion serves all
@@ -330,6 +292,72 @@ __global__ void __launch_bounds__(256) k_comb_tables_runs(const uint32_t *__rest
     load_aff(b, halves, COMB_HALF * i + 256 + h);
     pair_sums<F>(b, halves, COMB_HALF * i + r, S, tables, (i << 15) | (h << 8) | r, S,
                  COMB_RUN == 32 ? 0xffffffffu : (1u << COMB_RUN) - 1u);
+}
+
+// Step 2a: the half tables from four quarter tables per base (COMB_QUARTER entries, one thread
+// each, from the tooth bases):
+//   QA0_u = sum_{j<4} +-Q_j,  QA1_v = sum_{4<=j<8} +-Q_j,  QB0_u = sum_{j<4} +-Q_(8+j)  (16 each),
+//   QB1_v = Q_15 + sum_{j<3} +-Q_(12+j)  (8),
+// so A_l = QA0_(l mod 16) + QA1_(l >> 4) and B_h = QB0_(h mod 16) + QB1_(h >> 4).  For a point of
+// prime order the two terms are never equal or opposite (their multiples of P differ in size:
+// below 2^49 against at least 2^111, and below 2^177 against at least 2^239).
+constexpr uint32_t COMB_QUARTER = 56;
+template <class C>
+__global__ void __launch_bounds__(256) k_comb_quarters(const uint32_t *__restrict__ teeth, uint32_t n,
+                                                       uint32_t *__restrict__ quarters)
+{
+    using F = typename C::F;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * COMB_QUARTER)
+        return;
+    const uint32_t i = t / COMB_QUARTER, u = t % COMB_QUARTER;
+    const uint32_t part = u < 48 ? u >> 4 : 3, bits = u < 48 ? (u & 15u) : u - 48;
+    const int first = (int)(4 * part), count = part < 3 ? 4 : 3; // tooth bases of this quarter
+    aff q, a;
+    load_aff(q, teeth, 16 * i + 15);
+    bool pt = !aff_is_zero_words(q);
+    if (pt)
+    {
+        xyzz acc;
+        bool acc_id = true;
+        xyzz_identity<F>(acc);
+        if (part == 3)
+            xyzz_madd<F>(acc, acc_id, q);
+        for (int j = 0; j < count; j++)
+        {
+            load_aff(q, teeth, 16 * i + first + j);
+            if (!((bits >> j) & 1u))
+                fe_neg<F>(q.y, q.y);
+            xyzz_madd<F>(acc, acc_id, q);
+        }
+        pt = !acc_id && xyzz_to_aff<F>(a, acc);
+    }
+    if (!pt)
+    {
+        fe_zero(a.x);
+        fe_zero(a.y);
+    }
+    store_aff<F>(quarters, t, a);
+}
+
+// Step 2a, continued: thread (i, v) of base i forms 16 half entries, A_(16 v + k) for v < 16 and
+// B_(16 (v - 16) + k) for 16 <= v < 24.
+template <class C>
+__global__ void __launch_bounds__(256) k_comb_halves(const uint32_t *__restrict__ quarters, uint32_t n,
+                                                     uint32_t *__restrict__ halves)
+{
+    using F = typename C::F;
+    static_assert(COMB_RUN == 16, "one run per quarter-table row");
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 24u)
+        return;
+    const uint32_t i = t / 24u, v = t % 24u;
+    const bool hi = v >= 16;
+    const uint32_t q0 = COMB_QUARTER * i;
+    aff b;
+    load_aff(b, quarters, hi ? q0 + 48 + (v - 16) : q0 + 16 + v);
+    pair_sums<F>(b, quarters, hi ? q0 + 32 : q0, 1, halves, COMB_HALF * i + (hi ? 256 + 16 * (v - 16) : 16 * v), 1,
+                 0xffffu);
 }
 
 // Windowed tables (entries (k+1) P, k < E = 2^(WB-1), WB >= 8) by the same construction: with

@@ -1473,16 +1473,20 @@ template <class C> int fixed_build_comb(hs_gpu_fixed &f, hipStream_t st)
     constexpr int E = hs::FixedW<hs::FIXED_COMB16>::entries;
     int r;
     // the per-call scalar workspace stages the affine bases, then (once the teeth are built from
-    // them) the half tables
-    if ((r = f.tables.reserve_exact((size_t)f.n * E * 64)) || (r = f.scal.reserve((size_t)f.n * hs::COMB_HALF * 64)) ||
+    // them) the half tables, with the quarter tables behind them
+    if ((r = f.tables.reserve_exact((size_t)f.n * E * 64)) ||
+        (r = f.scal.reserve((size_t)f.n * (hs::COMB_HALF + hs::COMB_QUARTER) * 64)) ||
         (r = f.sums.reserve((size_t)f.n * 16 * 64)))
         return r;
+    uint32_t *quarters = f.scal.as<uint32_t>() + (size_t)f.n * hs::COMB_HALF * 16;
     hipLaunchKernelGGL(hs::k_fixed_bases<C>, dim3((f.n + 255) / 256), dim3(256), 0, st, f.points.as<uint64_t>(), f.n,
                        f.scal.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_comb_teeth<C>, dim3((16 * f.n + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(),
                        f.n, f.sums.as<uint32_t>());
-    const uint32_t th = f.n * hs::COMB_HALF, t = f.n * (128u * (256u / hs::COMB_RUN));
-    hipLaunchKernelGGL(hs::k_comb_halves<C>, dim3((th + 255) / 256), dim3(256), 0, st, f.sums.as<uint32_t>(), f.n,
+    const uint32_t tq = f.n * hs::COMB_QUARTER, th = f.n * 24u, t = f.n * (128u * (256u / hs::COMB_RUN));
+    hipLaunchKernelGGL(hs::k_comb_quarters<C>, dim3((tq + 255) / 256), dim3(256), 0, st, f.sums.as<uint32_t>(), f.n,
+                       quarters);
+    hipLaunchKernelGGL(hs::k_comb_halves<C>, dim3((th + 255) / 256), dim3(256), 0, st, quarters, f.n,
                        f.scal.as<uint32_t>());
     hipLaunchKernelGGL(hs::k_comb_tables_runs<C>, dim3((t + 255) / 256), dim3(256), 0, st, f.scal.as<uint32_t>(), f.n,
                        f.tables.as<uint32_t>());
