@@ -2939,8 +2939,24 @@ int hs_gpu_msm_bytes(int curve, unsigned char out32[32], const unsigned char *sc
         (r = cx.status.reserve(n * 32)) || (r = cx.out32.reserve(64)))
         return r;
     hipStream_t st = cx.stream;
-    HS_CHECK(hipMemcpyAsync(cx.scalars.p, scalars, n * 32, hipMemcpyHostToDevice, st));
-    HS_CHECK(hipMemcpyAsync(cx.status.p, points32, n * 32, hipMemcpyHostToDevice, st));
+    // the inputs go up through a staging slot's pinned buffers; the slot is returned on every exit
+    // (cx.mu is held throughout)
+    HostStage *hsg = take_stage(cx);
+    if (!hsg)
+        return HS_GPU_EDEVICE;
+    struct StageBack
+    {
+        Ctx &cx;
+        HostStage *h;
+        ~StageBack() { cx.stages_free.push_back(h); }
+    } back{cx, hsg};
+    r = hsg->upload(cx.scalars.p, scalars, n * 32, st);
+    if (!r)
+        r = hsg->upload(cx.status.p, points32, n * 32, st);
+    if (flags & HS_GPU_ERASE)
+        hsg->erase_pinned();
+    if (r)
+        return r;
     uint32_t nbad = 0;
     r = curve == HS_GPU_HELIOS
             ? decompress_device<hs::HeliosC>(cx, cx.status.as<uint32_t>(), n, cx.points.as<uint64_t>(), st, &nbad)
@@ -2950,8 +2966,14 @@ int hs_gpu_msm_bytes(int curve, unsigned char out32[32], const unsigned char *sc
     HS_CHECK(hipStreamSynchronize(st));
     if (nbad)
         return fail(HS_GPU_EDECODE, std::to_string(nbad) + " point(s) failed to decode");
-    if ((r = msm_device_any(curve, cx, cx.scalars.p, cx.points.p, n, flags, cx.out32.p, nullptr, st)))
+    r = msm_device_any(curve, cx, cx.scalars.p, cx.points.p, n, flags, cx.out32.p, nullptr, st);
+    if (flags & HS_GPU_ERASE)
+        zero_dev(cx.scalars.p, n * 32, st); // the device copy of the scalars
+    if (r)
+    {
+        hipStreamSynchronize(st);
         return r;
+    }
     HS_CHECK(hipMemcpyAsync(out32, cx.out32.p, 32, hipMemcpyDeviceToHost, st));
     HS_CHECK(hipStreamSynchronize(st));
     return 0;

@@ -213,6 +213,9 @@ def test_config2_shape_vs_compiled_reference(c, logn):
     pb = GH.ref_sswu_bases(c, 2018 + c, n, dup_frac=0.005, id_frac=0.001)
     want = O.ref_msm(c, sc, pb, n)
     assert hs.msm_bytes(c, sc, pb) == want
+    if logn >= 17:
+        # 8 MB inputs: the pinned staging path, wiped afterwards with ERASE
+        assert hs.msm_bytes(c, sc, pb, flags=hs.ERASE) == want
 
 
 @pytest.mark.parametrize("c", [0, 1])
